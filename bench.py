@@ -1,0 +1,192 @@
+"""Benchmark: ray-segments/s of one frame (trace + accumulate + resolve) of the large single
+scene on one MI355X, with the FP64 oracle's image error, per-kernel roofline, and the
+CPU restatement timed on the host cores beside it.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ray-segments/sec (1 GPU) at ≤1e-3 rel L1 image err; p50/p99 scene latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# Brute-force algorithmic FP32 flops per traced segment of the large scene: 16 circles x 18,
+# 16 rects x 28, 16 lenses x 56, 16 mirrors x 12, plus shading 40 and canvas exit 12.
+FLOPS_PER_SEGMENT_LARGE = 16 * (18 + 28 + 56 + 12) + 52
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return 0.0
+    k = (len(xs) - 1) * q / 100.0
+    f = int(k)
+    c = min(f + 1, len(xs) - 1)
+    return xs[f] + (xs[c] - xs[f]) * (k - f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the oracle image check")
+    ap.add_argument("--cpu-stride", type=int, default=3, help="CPU baseline traces every n-th ray")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (multi-GPU "
+              "sharding is a design note only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import light_garden_amd as lg
+    from light_garden_amd import abi, configs
+
+    if not os.path.exists(abi.LIB_PATH):
+        from light_garden_amd import build as lgb
+        lgb.build()
+
+    scene = configs.large_single(num_rays=args.rays)
+    ctx = lg.Context(0)
+    tr = lg.Tracer(ctx, scene.canvas)
+    for o in scene.objects:
+        tr.push_object(o)
+    for l in scene.lights:
+        tr.push_light(l)
+    tr.max_bounce = scene.max_bounce
+    segs = lg.Segments(ctx, capacity=8 * scene.n_rays())
+    img = lg.Image(ctx, scene.width, scene.height, scene.canvas)
+
+    for _ in range(args.warmup):
+        tr.render(segs, img)
+    ctx.synchronize()
+    n_seg = segs.count()
+
+    # timed region: K frames, per-stage HIP events recorded on the library's stream
+    ctx.set_profiling(True)
+    lat = []
+    stage_sum = {}
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        tr.render(segs, img)
+        lat.append((time.perf_counter() - t1) * 1e3)
+        for name, (ms, launches, work) in ctx.stage_times().items():
+            s = stage_sum.setdefault(name, [0.0, 0, 0])
+            s[0] += ms
+            s[1] += launches
+            s[2] += work
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n_seg * args.steps / elapsed
+
+    # dominant kernel and its roofline
+    stages = {k: {"ms_per_frame": v[0] / args.steps, "launches_per_frame": v[1] / args.steps,
+                  "avg_launch_ms": v[0] / max(1, v[1])} for k, v in stage_sum.items()}
+    dominant = max(stages, key=lambda k: stages[k]["ms_per_frame"])
+    px = scene.width * scene.height
+    # algorithmic bytes per launch: tile_accum reads each segment once (16 B end points +
+    # 16 B colour) and reads + writes the FP32 accumulator once (32 B per pixel)
+    algo_bytes = {"tile_accum": 32.0 * n_seg + 32.0 * px, "resolve": 24.0 * px, "clear": 16.0 * px,
+                  "bin_count": 16.0 * n_seg, "bin_fill": 16.0 * n_seg}
+    if dominant.startswith("trace"):
+        # per launch = one bounce generation; report the frame's FLOP rate over all launches
+        flops = FLOPS_PER_SEGMENT_LARGE * n_seg
+        t = stages[dominant]["ms_per_frame"] * 1e-3
+        achieved = flops / t / 1e12
+        roofline = {"bound": "valu", "kernel": dominant, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
+    else:
+        t = stages[dominant]["avg_launch_ms"] * 1e-3
+        b = algo_bytes.get(dominant, 0.0)
+        achieved = b / t / 1e9
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    trace_ms = sum(v["ms_per_frame"] for k, v in stages.items() if k.startswith("trace"))
+    trace_tflops = FLOPS_PER_SEGMENT_LARGE * n_seg / (trace_ms * 1e-3) / 1e12 if trace_ms else 0.0
+    ta = stages.get("tile_accum")
+    secondary = {
+        "trace_valu": {"achieved": trace_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": trace_tflops / FP32_PEAK_TFLOPS, "ms_per_frame": trace_ms},
+    }
+    if ta:
+        gbs = algo_bytes["tile_accum"] / (ta["avg_launch_ms"] * 1e-3) / 1e9
+        secondary["tile_accum_hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": gbs / HBM_PEAK_GBS, "ms_per_frame": ta["ms_per_frame"]}
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "ray-segments/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "large_single: %d rays (point + 30deg spot + 0.4 directional, seed 2), "
+                               "64 objects (16 circle/rect/lens/mirror), 10 bounces, %dx%d RGBA16F"
+                               % (scene.n_rays(), scene.width, scene.height),
+                   "segments_per_frame": n_seg, "trace": "fp32 + fp64 steps"},
+        "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "stages": stages,
+        "roofline": roofline,
+        "roofline_secondary": secondary,
+    }
+
+    if not args.no_check or not args.no_cpu:
+        from oracle import oracle as orc
+    if not args.no_check:
+        # image error of this frame vs the FP64 oracle's segments accumulated in FP64
+        threads = cpu_threads()
+        rpos, rcol, rkey = orc.trace_scene(scene, threads=threads)
+        ref = orc.accumulate_f64(rpos, rcol, None, scene.view(), threads=threads)
+        result["rel_l1_vs_fp64"] = orc.rel_l1(img.rgba(), ref)
+        result["oracle_segments"] = int(len(rkey))
+        del ref
+
+    if not args.no_cpu:
+        threads = cpu_threads()
+        stride = max(1, args.cpu_stride)
+        objs = lg.objects_array(scene.objects)
+        lights = lg.lights_array(scene.lights)
+        view = scene.view()
+        times = []
+        nseg_cpu = 0
+        for rep in range(4):  # 1 warm-up + 3 timed
+            acc = np.zeros((scene.height, scene.width, 4), np.float32)
+            t1 = time.perf_counter()
+            pos, col, key = orc.trace(objs, len(scene.objects), lights, len(scene.lights), scene.params(),
+                                      ray_stride=stride, threads=threads)
+            orc.accumulate_f32(pos.astype(np.float32), col, None, view, threads=threads, acc=acc)
+            dt = time.perf_counter() - t1
+            nseg_cpu = len(key)
+            if rep:
+                times.append(dt)
+        med = sorted(times)[len(times) // 2]
+        result["cpu_baseline"] = {
+            "value": nseg_cpu / med, "unit": "ray-segments/s", "cores": threads, "kind": "port",
+            "sample": "every %d-th primary ray of the same scene (%d segments): FP64 trace (chunks of "
+                      "100 rays) + FP32 accumulation, median of 3 after 1 warm-up, %.2f s each"
+                      % (stride, nseg_cpu, med),
+        }
+        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,236 @@
+/*
+ * lg.h — C ABI of the MI355X light_garden path: per-ray 2-D trace and additive line
+ * accumulation into an RGBA16F image.
+ *
+ * The reference is in-process Rust (sphereflow/light_garden) with no FFI. Its seam is
+ * the `Tracer` scene API (src/light_garden/tracer.rs:4-17, 48-116) and
+ * `Tracer::trace_all() -> Vec<(P2, Color)>` (tracer.rs:276), whose output is drawn by
+ * `SubRenderPass::update_vertex_buffer` + a LineList draw with additive blend into an
+ * Rgba16Float texture (src/sub_render_pass.rs:188-212, src/light_garden/mod.rs:57-73,
+ * src/texture_renderer.rs:5). Every entry point below names the reference item it
+ * replaces. The structs are plain data with fixed-width fields and explicit padding,
+ * so a Rust `#[repr(C)]` mirror binds them one to one.
+ *
+ * Conventions
+ *   - Every call returns an int status (LG_OK == 0). lg_last_error() gives a message for
+ *     the last failing call on the calling thread.
+ *   - Host arrays passed in are copied during the call; the caller keeps ownership.
+ *   - Device buffers behind lg_segments / lg_image belong to the library until the
+ *     matching *_destroy.
+ *   - One host thread per lg_ctx at a time. Each context owns one HIP stream; separate
+ *     contexts are independent. A scene must not be edited while a trace of it runs
+ *     (the reference's `&self` trace / `&mut self` edit split, tracer.rs:360).
+ */
+#ifndef LG_H
+#define LG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LG_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------------- */
+enum lg_status {
+  LG_OK = 0,
+  LG_E_INVALID = 1,     /* malformed argument or scene                              */
+  LG_E_OOM = 2,         /* device allocation failed or memory budget exceeded        */
+  LG_E_CAPACITY = 3,    /* caller's output array too small; *needed says how large   */
+  LG_E_HIP = 4,         /* HIP runtime error                                         */
+  LG_E_UNSUPPORTED = 5, /* valid in the reference but not handled by this build      */
+  LG_E_NODEVICE = 6,    /* no usable gfx950 device                                   */
+  LG_E_TIMEOUT = 7,     /* serving: request deadline passed                          */
+  LG_E_NOTFOUND = 8     /* serving: unknown request or scene id                       */
+};
+
+/* ---- geometry (collision2d::Geo variants reachable from ObjectE, object.rs:6-16) -- */
+enum lg_shape_kind {
+  LG_SHAPE_LINE_SEGMENT = 0,   /* Geo::GeoLineSegment — StraightMirror (object.rs:315)  */
+  LG_SHAPE_CUBIC_BEZIER = 1,   /* Geo::GeoCubicBezier — CurvedMirror (object.rs:345)    */
+  LG_SHAPE_CIRCLE = 2,         /* Geo::GeoCircle (object.rs:287)                        */
+  LG_SHAPE_RECT = 3,           /* Geo::GeoRect, rotated (object.rs:288)                 */
+  LG_SHAPE_LOGIC = 4,          /* Geo::GeoLogic — Lens and boolean ops (object.rs:387)  */
+  LG_SHAPE_CONVEX_POLYGON = 5, /* Geo::GeoConvexPolygon (object.rs:290)                 */
+  LG_SHAPE_ELLIPSE = 6         /* Geo::GeoEllipse (object.rs:291)                       */
+};
+
+enum lg_logic_op { LG_LOGIC_AND = 0, LG_LOGIC_OR = 1, LG_LOGIC_AND_NOT = 2 };
+
+#define LG_MAX_SHAPES 7          /* nodes of one object's Logic tree (depth <= 2)    */
+#define LG_MAX_POLY_VERTICES 16  /* vertices of all polygons of one object            */
+
+/*
+ * One node of an object's shape tree. Node 0 is the root. A node's frame is
+ * world = parent_frame(origin + R * local), R column-major [m00, m10, m01, m11]
+ * (the RON `rotation: [..4]` layout, default.ron:24-29).
+ *   LINE_SEGMENT   p = {ax, ay, bx, by}                         (origin/rotation unused)
+ *   CUBIC_BEZIER   p = {p0x, p0y, p1x, p1y, p2x, p2y, p3x, p3y} (origin/rotation unused)
+ *   CIRCLE         origin = centre, p[0] = radius
+ *   RECT           origin = centre, rotation, p[0] = width, p[1] = height
+ *   ELLIPSE        origin = centre, rotation, p[0] = a (local x semi-axis), p[1] = b
+ *   CONVEX_POLYGON origin, rotation; vertices (local, counter-clockwise) at
+ *                  lg_object.vertices[2*vertex_offset ..], n_vertices of them
+ *   LOGIC          origin, rotation = local frame of the two operands; op,
+ *                  child_a / child_b = node indices (> this node's index)
+ */
+typedef struct lg_shape {
+  int32_t kind;
+  int32_t op;
+  int32_t child_a;
+  int32_t child_b;
+  int32_t n_vertices;
+  int32_t vertex_offset;
+  double origin[2];
+  double rotation[4];
+  double p[8];
+} lg_shape;
+
+/* Object {object_enum, material_opt: Option<Material>, moved} (object.rs:50-55). */
+typedef struct lg_object {
+  int32_t has_material;      /* material_opt.is_some(); without it the object mirrors */
+  int32_t n_shapes;          /* nodes used in shapes[]                                 */
+  double refractive_index;   /* Material::refractive_index (default 1.2, object.rs:444) */
+  lg_shape shapes[LG_MAX_SHAPES];
+  double vertices[2 * LG_MAX_POLY_VERTICES];
+} lg_object;
+
+/* ---- lights (light.rs:9-14) -------------------------------------------------------- */
+enum lg_light_kind {
+  LG_LIGHT_POINT = 0,        /* PointLight{position, color, num_rays}  light.rs:139       */
+  LG_LIGHT_DIRECTIONAL = 1,  /* DirectionalLight{color, num_rays, start} light.rs:79      */
+  LG_LIGHT_SPOT = 2          /* SpotLight{position, color, num_rays, spot_angle, dir} 190 */
+};
+
+typedef struct lg_light {
+  int32_t kind;
+  int32_t _pad0;
+  uint64_t num_rays;
+  float color[4];            /* Color = [f32; 4] (light.rs:7), RGB plus alpha            */
+  double position[2];        /* point/spot: position; directional: band start a          */
+  double direction[2];       /* spot: spot_direction; directional: band end b            */
+  double spot_angle;         /* spot: full cone angle in radians                          */
+} lg_light;
+
+/* ---- trace parameters (Tracer fields, tracer.rs:9-13, 37-39) ------------------------ */
+enum lg_trace_flags {
+  LG_TRACE_FP64 = 1u << 0    /* trace in FP64 end to end instead of FP32 + FP64 islands */
+};
+
+typedef struct lg_params {
+  uint32_t max_bounce;       /* generations traced (default 5)                          */
+  uint32_t flags;            /* lg_trace_flags                                           */
+  float cutoff_color[4];     /* default {0.001, 0.001, 0.001, 0.001}                     */
+  double canvas[4];          /* canvas_bounds as {left, right, bottom, top}               */
+} lg_params;
+
+/* Image view: world rect projected onto width x height pixels, row 0 at the top
+ * (ortho(-a, a, -1, 1) of renderer.rs:120-124 generalised to any rect). */
+typedef struct lg_view {
+  double left, right, bottom, top;
+  uint32_t width, height;
+} lg_view;
+
+/* ---- opaque handles ------------------------------------------------------------------ */
+typedef struct lg_ctx lg_ctx;
+typedef struct lg_scene lg_scene;
+typedef struct lg_segments lg_segments;
+typedef struct lg_image lg_image;
+
+/* ---- context ------------------------------------------------------------------------- */
+const char* lg_last_error(void);
+int lg_version(void);
+/* Opens `device` (HIP ordinal), creates the context's stream and scratch arenas. */
+int lg_ctx_create(int device, lg_ctx** out);
+/* Releases the context; while scenes, segment buffers or images made from it are alive the
+ * release waits for the last of them to be destroyed. */
+int lg_ctx_destroy(lg_ctx* ctx);
+int lg_ctx_synchronize(lg_ctx* ctx);
+/* Per-stage HIP-event timing of the context's next calls (off by default). */
+int lg_ctx_set_profiling(lg_ctx* ctx, int enable);
+typedef struct lg_stage_time {
+  char name[24];
+  float ms;                  /* sum over the stage's launches in the last profiled call */
+  uint32_t launches;
+  uint64_t work;             /* stage-specific unit count (segments, pairs, pixels)     */
+} lg_stage_time;
+int lg_ctx_stage_times(lg_ctx* ctx, lg_stage_time* out, int cap, int* n_out);
+
+/* ---- scene: the Tracer's object/light container (tracer.rs:4-17, 48-116) ------------- */
+int lg_scene_create(lg_ctx* ctx, lg_scene** out);
+int lg_scene_destroy(lg_scene* scene);
+/* Tracer::clear (tracer.rs:54) / clear_objects (tracer.rs:48). */
+int lg_scene_clear(lg_scene* scene);
+int lg_scene_clear_objects(lg_scene* scene);
+/* Tracer::push_object (tracer.rs:87) / push_light (tracer.rs:92). */
+int lg_scene_push_object(lg_scene* scene, const lg_object* obj);
+int lg_scene_push_light(lg_scene* scene, const lg_light* light);
+/* Bulk forms of push_*: replace all objects / all lights (Tracer::load, tracer.rs:190). */
+int lg_scene_set_objects(lg_scene* scene, const lg_object* objs, uint32_t n);
+int lg_scene_set_lights(lg_scene* scene, const lg_light* lights, uint32_t n);
+/* Tracer::replace_object (tracer.rs:104), remove_object (109), remove_light (114). */
+int lg_scene_replace_object(lg_scene* scene, uint32_t ix, const lg_object* obj);
+int lg_scene_remove_object(lg_scene* scene, uint32_t ix);
+int lg_scene_remove_light(lg_scene* scene, uint32_t ix);
+int lg_scene_counts(const lg_scene* scene, uint32_t* n_objects, uint32_t* n_lights,
+                    uint64_t* n_rays);
+
+/* ---- segments: the `Vec<(P2, Color)>` vertex pairs of trace_all (tracer.rs:451-486) --- */
+/* Device SoA: pos float4 {x0,y0,x1,y1}; color0 float4; optional color1 float4 (per-vertex
+ * colours, string_mod.rs:118-119); key uint64 (ray index << 24 | path bits). */
+enum lg_segment_flags { LG_SEG_VERTEX_COLORS = 1u << 0 };
+int lg_segments_create(lg_ctx* ctx, uint64_t capacity, uint32_t flags, lg_segments** out);
+int lg_segments_destroy(lg_segments* segs);
+int lg_segments_count(const lg_segments* segs, uint64_t* count);
+/* Host -> device (replaces update_vertex_buffer's upload, sub_render_pass.rs:188-203).
+ * color1 may be NULL unless the buffer was created with LG_SEG_VERTEX_COLORS. */
+int lg_segments_upload(lg_segments* segs, const float* pos, const float* color0,
+                       const float* color1, uint64_t n);
+/* Device -> host; any output pointer may be NULL. Fails with LG_E_CAPACITY if cap < count. */
+int lg_segments_download(lg_segments* segs, float* pos, float* color0, float* color1,
+                         uint64_t* key, uint64_t cap, uint64_t* count);
+
+/* FP64 end points {x0,y0,x1,y1} of the last LG_TRACE_FP64 trace into `out` (the
+ * reference's P2 = Point2<f64> vertices, tracer.rs:451-452). */
+int lg_segments_download_pos64(lg_segments* segs, double* pos64, uint64_t cap, uint64_t* count);
+
+/* ---- trace: Tracer::trace_all / Tracer::trace (tracer.rs:276-358, 360-493) ------------ */
+/* Traces every light of the scene into `out` (device). Segment order in the buffer is
+ * unspecified; the key gives each segment's identity and the reference's emission order. */
+int lg_trace(lg_scene* scene, const lg_params* params, lg_segments* out);
+/* Drop-in for trace_all's return value: vertex pairs in the reference's order
+ * (light, ray, depth, parent, reflected before refracted). xy gets 2 doubles and rgba 4
+ * floats per vertex, 2 vertices per segment. Fails with LG_E_CAPACITY and *n_vertices set
+ * to the needed count when cap_vertices is too small. */
+int lg_trace_all(lg_scene* scene, const lg_params* params, double* xy, float* rgba,
+                 uint64_t cap_vertices, uint64_t* n_vertices);
+
+/* ---- image: the Rgba16Float render target (texture_renderer.rs:5, 69-80) --------------- */
+int lg_image_create(lg_ctx* ctx, const lg_view* view, lg_image** out);
+int lg_image_destroy(lg_image* img);
+/* Clears the FP32 accumulator to zero (the resolve adds the target's clear alpha of 1,
+ * renderer.rs:175). */
+int lg_image_clear(lg_image* img);
+/* Device pointers for zero-copy interop: accum is float4 per pixel (row-major), rgba16f is
+ * 4 half per pixel. */
+int lg_image_device_ptrs(lg_image* img, void** accum, void** rgba16f);
+int lg_image_download_rgba16f(lg_image* img, uint16_t* host, uint64_t cap_halves);
+int lg_image_download_accum(lg_image* img, float* host, uint64_t cap_floats);
+
+/* ---- accumulation: LineList draw + One/One blend (shader.wgsl:14-28, mod.rs:57-73) ---- */
+/* Adds every segment into img's FP32 accumulator: rgb += w*rgb, a += w*a*a, with the
+ * anti-aliased column/row sampling rule of this build. */
+int lg_accumulate(lg_ctx* ctx, const lg_segments* segs, lg_image* img);
+/* FP32 accumulator -> RGBA16F target (round to nearest even), alpha += 1 (clear value). */
+int lg_resolve(lg_ctx* ctx, lg_image* img);
+/* One frame: clear, trace, accumulate, resolve (LightGarden::draw + Renderer::render,
+ * mod.rs:680-696, renderer.rs:420). `segs` receives the trace output. */
+int lg_render(lg_scene* scene, const lg_params* params, lg_segments* segs, lg_image* img);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LG_H */

@@ -1,0 +1,174 @@
+"""ctypes mirror of include/lg.h and the loader for the gfx950 library.
+
+The library is light_garden_amd/build/liblg.so, built by light_garden_amd/build.py. There is
+no fallback: if the library is missing or cannot open a gfx950 device, calls raise.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liblg.so")
+
+# ---- constants (lg.h) --------------------------------------------------------------------
+LG_OK = 0
+LG_E_INVALID = 1
+LG_E_OOM = 2
+LG_E_CAPACITY = 3
+LG_E_HIP = 4
+LG_E_UNSUPPORTED = 5
+LG_E_NODEVICE = 6
+LG_E_TIMEOUT = 7
+LG_E_NOTFOUND = 8
+STATUS_NAMES = {0: "LG_OK", 1: "LG_E_INVALID", 2: "LG_E_OOM", 3: "LG_E_CAPACITY", 4: "LG_E_HIP",
+                5: "LG_E_UNSUPPORTED", 6: "LG_E_NODEVICE", 7: "LG_E_TIMEOUT", 8: "LG_E_NOTFOUND"}
+
+SHAPE_LINE_SEGMENT = 0
+SHAPE_CUBIC_BEZIER = 1
+SHAPE_CIRCLE = 2
+SHAPE_RECT = 3
+SHAPE_LOGIC = 4
+SHAPE_CONVEX_POLYGON = 5
+SHAPE_ELLIPSE = 6
+
+LOGIC_AND = 0
+LOGIC_OR = 1
+LOGIC_AND_NOT = 2
+
+LIGHT_POINT = 0
+LIGHT_DIRECTIONAL = 1
+LIGHT_SPOT = 2
+
+TRACE_FP64 = 1
+SEG_VERTEX_COLORS = 1
+
+MAX_SHAPES = 7
+MAX_POLY_VERTICES = 16
+
+
+class LgShape(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("op", C.c_int32), ("child_a", C.c_int32), ("child_b", C.c_int32),
+        ("n_vertices", C.c_int32), ("vertex_offset", C.c_int32),
+        ("origin", C.c_double * 2), ("rotation", C.c_double * 4), ("p", C.c_double * 8),
+    ]
+
+
+class LgObject(C.Structure):
+    _fields_ = [
+        ("has_material", C.c_int32), ("n_shapes", C.c_int32), ("refractive_index", C.c_double),
+        ("shapes", LgShape * MAX_SHAPES), ("vertices", C.c_double * (2 * MAX_POLY_VERTICES)),
+    ]
+
+
+class LgLight(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("_pad0", C.c_int32), ("num_rays", C.c_uint64),
+        ("color", C.c_float * 4), ("position", C.c_double * 2), ("direction", C.c_double * 2),
+        ("spot_angle", C.c_double),
+    ]
+
+
+class LgParams(C.Structure):
+    _fields_ = [
+        ("max_bounce", C.c_uint32), ("flags", C.c_uint32), ("cutoff_color", C.c_float * 4),
+        ("canvas", C.c_double * 4),
+    ]
+
+
+class LgView(C.Structure):
+    _fields_ = [
+        ("left", C.c_double), ("right", C.c_double), ("bottom", C.c_double), ("top", C.c_double),
+        ("width", C.c_uint32), ("height", C.c_uint32),
+    ]
+
+
+class LgStageTime(C.Structure):
+    _fields_ = [("name", C.c_char * 24), ("ms", C.c_float), ("launches", C.c_uint32),
+                ("work", C.c_uint64)]
+
+
+assert C.sizeof(LgShape) == 136
+assert C.sizeof(LgObject) == 16 + 7 * 136 + 256
+assert C.sizeof(LgLight) == 72
+assert C.sizeof(LgParams) == 56
+assert C.sizeof(LgView) == 40
+
+# Every function lg.h declares: name -> (restype, argtypes)
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+FUNCTIONS = {
+    "lg_last_error": (C.c_char_p, []),
+    "lg_version": (C.c_int, []),
+    "lg_ctx_create": (C.c_int, [C.c_int, _PP]),
+    "lg_ctx_destroy": (C.c_int, [_P]),
+    "lg_ctx_synchronize": (C.c_int, [_P]),
+    "lg_ctx_set_profiling": (C.c_int, [_P, C.c_int]),
+    "lg_ctx_stage_times": (C.c_int, [_P, C.POINTER(LgStageTime), C.c_int, C.POINTER(C.c_int)]),
+    "lg_scene_create": (C.c_int, [_P, _PP]),
+    "lg_scene_destroy": (C.c_int, [_P]),
+    "lg_scene_clear": (C.c_int, [_P]),
+    "lg_scene_clear_objects": (C.c_int, [_P]),
+    "lg_scene_push_object": (C.c_int, [_P, C.POINTER(LgObject)]),
+    "lg_scene_push_light": (C.c_int, [_P, C.POINTER(LgLight)]),
+    "lg_scene_set_objects": (C.c_int, [_P, C.POINTER(LgObject), C.c_uint32]),
+    "lg_scene_set_lights": (C.c_int, [_P, C.POINTER(LgLight), C.c_uint32]),
+    "lg_scene_replace_object": (C.c_int, [_P, C.c_uint32, C.POINTER(LgObject)]),
+    "lg_scene_remove_object": (C.c_int, [_P, C.c_uint32]),
+    "lg_scene_remove_light": (C.c_int, [_P, C.c_uint32]),
+    "lg_scene_counts": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint64)]),
+    "lg_segments_create": (C.c_int, [_P, C.c_uint64, C.c_uint32, _PP]),
+    "lg_segments_destroy": (C.c_int, [_P]),
+    "lg_segments_count": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "lg_segments_upload": (C.c_int, [_P, _P, _P, _P, C.c_uint64]),
+    "lg_segments_download": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "lg_segments_download_pos64": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "lg_trace": (C.c_int, [_P, C.POINTER(LgParams), _P]),
+    "lg_trace_all": (C.c_int, [_P, C.POINTER(LgParams), _P, _P, C.c_uint64,
+                               C.POINTER(C.c_uint64)]),
+    "lg_image_create": (C.c_int, [_P, C.POINTER(LgView), _PP]),
+    "lg_image_destroy": (C.c_int, [_P]),
+    "lg_image_clear": (C.c_int, [_P]),
+    "lg_image_device_ptrs": (C.c_int, [_P, _PP, _PP]),
+    "lg_image_download_rgba16f": (C.c_int, [_P, _P, C.c_uint64]),
+    "lg_image_download_accum": (C.c_int, [_P, _P, C.c_uint64]),
+    "lg_accumulate": (C.c_int, [_P, _P, _P]),
+    "lg_resolve": (C.c_int, [_P, _P]),
+    "lg_render": (C.c_int, [_P, C.POINTER(LgParams), _P, _P]),
+}
+
+_lib = None
+
+
+class LgError(RuntimeError):
+    def __init__(self, code, where, msg):
+        super().__init__("%s failed with %s: %s" % (where, STATUS_NAMES.get(code, code), msg))
+        self.code = code
+
+
+def load():
+    """Opens liblg.so (built in-tree). Raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("light_garden_amd: %s is missing; run light_garden_amd/build.py "
+                           "(or __graft_entry__.build())" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in FUNCTIONS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code, where):
+    if code != LG_OK:
+        msg = load().lg_last_error()
+        raise LgError(code, where, msg.decode() if msg else "")
+    return code
+
+
+def call(name, *args):
+    return check(getattr(load(), name)(*args), name)

@@ -1,0 +1,195 @@
+// lg_internal.h — layouts shared by the host side (lg_api.hip) and the kernels
+// (lg_trace.hip, lg_accum.hip). gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lg.h"
+
+namespace lg {
+
+// ---- trace constants ------------------------------------------------------------------
+// Hits with t <= kTMin along a ray are ignored: a child ray starts on the surface it left.
+constexpr double kTMin = 1e-5;
+// Spot lights with |spot_direction.x| below this point straight up or down (light.rs:230).
+constexpr double kEpsilon = 1e-9;
+// Path bits kept in a segment key (ray_index << kPathBits | path).
+constexpr int kPathBits = 24;
+constexpr uint32_t kMaxBounce = kPathBits - 1;
+
+// ---- device scene records ------------------------------------------------------------
+// One record per shape in object order; a Logic record is followed by its two operand
+// records. Everything is in world coordinates.
+enum RecKind : int32_t {
+  REC_SEG = LG_SHAPE_LINE_SEGMENT,      // p: ax ay ex ey nx ny   (e = b - a, n = left unit normal)
+  REC_BEZ = LG_SHAPE_CUBIC_BEZIER,      // p: a3 a2 a1 a0 (power basis, x/y pairs)
+  REC_CIRCLE = LG_SHAPE_CIRCLE,         // p: cx cy r r^2
+  REC_RECT = LG_SHAPE_RECT,             // p: cx cy ux uy vx vy hw hh
+  REC_LOGIC = LG_SHAPE_LOGIC,           // op = lg_logic_op; operands at +1, +2
+  REC_POLY = LG_SHAPE_CONVEX_POLYGON,   // p: cx cy ux uy vx vy; aux = first vertex, op = count
+  REC_ELLIPSE = LG_SHAPE_ELLIPSE        // p: cx cy ux uy vx vy a b
+};
+
+template <typename T>
+struct alignas(16) Rec {
+  int32_t kind;
+  int32_t obj;
+  int32_t op;
+  int32_t aux;
+  T p[12];
+};
+static_assert(sizeof(Rec<float>) == 64, "Rec<float> is four float4");
+
+// ---- containment (strict; open curves contain nothing), host and device -------------
+template <typename T>
+__host__ __device__ inline T abs_hd(T x) {
+  return x < (T)0 ? -x : x;
+}
+
+template <typename T>
+__host__ __device__ inline bool contains_prim(const Rec<T>& R, const T* verts, T px, T py) {
+  switch (R.kind) {
+    case REC_CIRCLE: {
+      T x = px - R.p[0], y = py - R.p[1];
+      return x * x + y * y < R.p[3];
+    }
+    case REC_RECT: {
+      T lx = px - R.p[0], ly = py - R.p[1];
+      T x = lx * R.p[2] + ly * R.p[3], y = lx * R.p[4] + ly * R.p[5];
+      return abs_hd(x) < R.p[6] && abs_hd(y) < R.p[7];
+    }
+    case REC_ELLIPSE: {
+      T lx = px - R.p[0], ly = py - R.p[1];
+      T x = (lx * R.p[2] + ly * R.p[3]) / R.p[6], y = (lx * R.p[4] + ly * R.p[5]) / R.p[7];
+      return x * x + y * y < (T)1;
+    }
+    case REC_POLY: {
+      T lx = px - R.p[0], ly = py - R.p[1];
+      T x = lx * R.p[2] + ly * R.p[3], y = lx * R.p[4] + ly * R.p[5];
+      int first = R.aux, nv = R.op;
+      if (nv < 3) return false;
+      for (int i = 0; i < nv; i++) {
+        int j = (i + 1 == nv) ? 0 : i + 1;
+        T ax = verts[2 * (first + i)], ay = verts[2 * (first + i) + 1];
+        T bx = verts[2 * (first + j)], by = verts[2 * (first + j) + 1];
+        T cr = (bx - ax) * (y - ay) - (by - ay) * (x - ax);
+        if (!(cr > (T)0)) return false;
+      }
+      return true;
+    }
+    default: return false;
+  }
+}
+
+// Object::contains (object.rs:240-259) over the object's records starting at `first`.
+template <typename T>
+__host__ __device__ inline bool contains_object(const Rec<T>* recs, const T* verts, int first,
+                                                T px, T py) {
+  const Rec<T>& R = recs[first];
+  if (R.kind == REC_LOGIC) {
+    bool ca = contains_prim(recs[first + 1], verts, px, py);
+    bool cb = contains_prim(recs[first + 2], verts, px, py);
+    if (R.op == LG_LOGIC_AND) return ca && cb;
+    if (R.op == LG_LOGIC_OR) return ca || cb;
+    return ca && !cb;
+  }
+  return contains_prim(R, verts, px, py);
+}
+
+// Per-object flags.
+enum ObjFlags : uint32_t { OBJ_MATERIAL = 1u, OBJ_CLOSED = 2u };
+
+constexpr int kMaxRecords = 1024;      // records resident in LDS per trace workgroup
+constexpr int kMaxObjects = 1024;
+constexpr int kMaxPolyVertices = 1024; // vertices of all polygons in a scene
+constexpr int kMaxLights = 64;
+
+// ---- lights --------------------------------------------------------------------------
+struct DevLight {
+  int32_t kind;
+  int32_t _pad;
+  uint64_t num_rays;
+  uint64_t ray_base;    // global index of the light's first ray
+  float color[4];
+  double a[2];          // point/spot position; directional band start
+  double e[2];          // directional: b - a
+  double nrm[2];        // directional: unit normal
+  double min_angle;     // spot
+  double spot_angle;    // spot
+  double sign;          // spot: f64::signum(spot_direction.x)
+  double ior;           // starting refractive index (tracer.rs:280-287)
+};
+
+// ---- rays in flight -------------------------------------------------------------------
+template <typename T>
+struct alignas(16) QRay {
+  T o[2];
+  T d[2];
+  float c[4];
+  T ior;
+  uint64_t key;
+};
+
+// ---- trace launch arguments -----------------------------------------------------------
+template <typename T>
+struct TraceArgs {
+  const Rec<T>* recs;         // culling records (precision T), loaded into LDS
+  const Rec<double>* recs64;  // FP64 records: re-evaluation of winners and shading
+  const double* obj_n64;      // refractive index per object
+  const uint32_t* obj_flags;
+  const int32_t* obj_rec;     // first record of each object
+  const T* verts;             // polygon vertices (local frame)
+  const double* verts64;
+  int32_t n_recs;
+  int32_t n_objs;
+  int32_t n_verts;
+  int32_t n_lights;
+  const DevLight* lights;
+  uint64_t n_primary;         // generation 0: rays to generate
+  uint32_t gen;
+  uint32_t last_gen;          // 1: children are not kept (tracer.rs:373, 490)
+  const QRay<double>* in_q;
+  const uint32_t* in_count;   // generation >= 1: device count of in_q
+  QRay<double>* out_q;
+  uint32_t* out_count;
+  uint32_t q_cap;
+  float cutoff[4];
+  double canvas[4];           // left right bottom top
+  // segment output
+  float4* seg_pos;
+  float4* seg_col;
+  uint64_t* seg_key;
+  double* seg_pos64;          // optional, FP64 trace only (4 per segment)
+  unsigned long long* seg_count;
+  uint64_t seg_cap;
+};
+
+// ---- accumulation ---------------------------------------------------------------------
+struct AccumView {
+  double left, top;
+  double sx, sy;              // pixels per world unit
+  int32_t width, height;
+  int32_t tiles_x, tiles_y;
+};
+
+struct WorkItem {
+  uint32_t tile;
+  uint32_t split;             // 1: other items share the tile -> flush with atomics
+  uint32_t begin, end;        // range in the pair list
+};
+
+// ---- error helpers (host) -------------------------------------------------------------
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+}  // namespace lg
+
+#define LG_HIP(call)                                   \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return lg::hip_fail(_e, #call); \
+  } while (0)

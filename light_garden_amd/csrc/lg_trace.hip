@@ -1,0 +1,958 @@
+// lg_trace.hip — the per-ray trace on gfx950.
+//
+// Replaces Tracer::trace_all / Tracer::trace (src/light_garden/tracer.rs:276-330,
+// 360-493) and the collision2d ray/shape arithmetic it calls. The reference walks each
+// primary ray's bounce tree breadth-first on a rayon worker; here one kernel launch is one
+// generation of ALL rays: each lane owns one live ray, tests it against every scene record
+// held in LDS (records are uniform across the wave, so the object loop never diverges),
+// shades the nearest hit, and appends its segment and surviving children with one global
+// atomic per workgroup (wave64 ballot + mbcnt prefix, then a cross-wave LDS prefix).
+// Children that the cutoff (tracer.rs:378-384) would skip are never queued, which is the
+// same as queueing and skipping them.
+//
+// Precision: T = float is the bulk path. The steps whose branch decides the ray tree are
+// redone in FP64 when FP32 cannot decide them: the circle/ellipse discriminant near
+// tangency, the Bezier sign tests near a double root or an end point, and the Snell
+// critical-angle (total internal reflection) test. T = double runs the whole trace in
+// FP64.
+#include <float.h>
+
+#include "lg_internal.h"
+
+namespace lg {
+
+template <typename T>
+struct RayT {
+  T ox, oy, dx, dy;
+};
+
+template <typename T>
+__device__ __forceinline__ T tmin_v() {
+  return (T)kTMin;
+}
+template <typename T>
+__device__ __forceinline__ T inf_v() {
+  return (T)INFINITY;
+}
+template <typename T>
+__device__ __forceinline__ T sqrt_v(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_sqrtf(x);
+  else
+    return __builtin_sqrt(x);
+}
+template <typename T>
+__device__ __forceinline__ T abs_v(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_fabsf(x);
+  else
+    return __builtin_fabs(x);
+}
+template <typename T>
+__device__ __forceinline__ T rcp_v(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_rcpf(x);
+  else
+    return (T)1 / x;
+}
+template <typename T>
+__device__ __forceinline__ void normalize_v(T& x, T& y) {
+  T l2 = x * x + y * y;
+  if (l2 > (T)0) {
+    if constexpr (sizeof(T) == 4) {
+      T r = __builtin_amdgcn_rsqf(l2);
+      x *= r;
+      y *= r;
+    } else {
+      T l = sqrt_v(l2);
+      x /= l;
+      y /= l;
+    }
+  }
+}
+
+// FP32 -> FP64 step threshold: a branch quantity q is recomputed in FP64 when
+// |q| < kIsland * (magnitude of the terms it was computed from).
+constexpr float kIsland = 1.0e-5f;
+
+// ---------------------------------------------------------------------------------------
+// chords of closed convex primitives: the parameter interval [tin, tout] the full line
+// o + t d spends inside (tin may be negative). Returns false when the line misses.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ bool chord_circle(const RayT<T>& r, const T* p, const Rec<double>* r64, T& tin,
+                             T& tout) {
+  T lx = r.ox - p[0], ly = r.oy - p[1];
+  T b = lx * r.dx + ly * r.dy;
+  T ll = lx * lx + ly * ly;
+  T cc = ll - p[3];
+  T disc = b * b - cc;
+  if constexpr (sizeof(T) == 4) {
+    if (__builtin_fabsf(disc) < kIsland * (b * b + ll + p[3])) {
+      RayT<double> rd{(double)r.ox, (double)r.oy, (double)r.dx, (double)r.dy};
+      double ti, to;
+      bool ok = chord_circle<double>(rd, r64->p, nullptr, ti, to);
+      tin = (float)ti;
+      tout = (float)to;
+      return ok;
+    }
+  }
+  if (disc < (T)0) return false;
+  T sq = sqrt_v(disc);
+  T q = b > (T)0 ? -b - sq : -b + sq;
+  if (q == (T)0) {
+    tin = tout = (T)0;
+    return true;
+  }
+  T ra = q, rb = cc / q;
+  tin = ra < rb ? ra : rb;
+  tout = ra < rb ? rb : ra;
+  return true;
+}
+
+// Local coordinates of the ray in a frame {c, u, v} (p[0..5]).
+template <typename T>
+__device__ __forceinline__ void to_local(const RayT<T>& r, const T* p, T& ox, T& oy, T& dx,
+                                         T& dy) {
+  T lx = r.ox - p[0], ly = r.oy - p[1];
+  ox = lx * p[2] + ly * p[3];
+  oy = lx * p[4] + ly * p[5];
+  dx = r.dx * p[2] + r.dy * p[3];
+  dy = r.dx * p[4] + r.dy * p[5];
+}
+
+// Rect faces: 0 bottom (0,-1), 1 right (1,0), 2 top (0,1), 3 left (-1,0).
+template <typename T>
+__device__ bool chord_rect(const RayT<T>& r, const T* p, T& tin, T& tout, int& fin, int& fout) {
+  T ox, oy, dx, dy;
+  to_local(r, p, ox, oy, dx, dy);
+  T hw = p[6], hh = p[7];
+  T t0 = -inf_v<T>(), t1 = inf_v<T>();
+  int f0 = -1, f1 = -1;
+  if (dx != (T)0) {
+    T inv = rcp_v(dx);
+    T ta = (-hw - ox) * inv, tb = (hw - ox) * inv;
+    if (ta < tb) {
+      t0 = ta; f0 = 3; t1 = tb; f1 = 1;
+    } else {
+      t0 = tb; f0 = 1; t1 = ta; f1 = 3;
+    }
+  } else if (ox < -hw || ox > hw) {
+    return false;
+  }
+  if (dy != (T)0) {
+    T inv = rcp_v(dy);
+    T ta = (-hh - oy) * inv, tb = (hh - oy) * inv;
+    T lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
+    int flo = ta < tb ? 0 : 2, fhi = ta < tb ? 2 : 0;
+    if (lo > t0) { t0 = lo; f0 = flo; }
+    if (hi < t1) { t1 = hi; f1 = fhi; }
+  } else if (oy < -hh || oy > hh) {
+    return false;
+  }
+  if (!(t0 <= t1)) return false;
+  tin = t0; tout = t1; fin = f0; fout = f1;
+  return true;
+}
+
+template <typename T>
+__device__ bool chord_ellipse(const RayT<T>& r, const T* p, const Rec<double>* r64, T& tin,
+                              T& tout) {
+  T ox, oy, dx, dy;
+  to_local(r, p, ox, oy, dx, dy);
+  T ia = (T)1 / p[6], ib = (T)1 / p[7];
+  T osx = ox * ia, osy = oy * ib, dsx = dx * ia, dsy = dy * ib;
+  T A = dsx * dsx + dsy * dsy, B = osx * dsx + osy * dsy, C = osx * osx + osy * osy - (T)1;
+  T disc = B * B - A * C;
+  if constexpr (sizeof(T) == 4) {
+    if (__builtin_fabsf(disc) < kIsland * (B * B + __builtin_fabsf(A * C) + A)) {
+      RayT<double> rd{(double)r.ox, (double)r.oy, (double)r.dx, (double)r.dy};
+      double ti, to;
+      bool ok = chord_ellipse<double>(rd, r64->p, nullptr, ti, to);
+      tin = (float)ti;
+      tout = (float)to;
+      return ok;
+    }
+  }
+  if (disc < (T)0 || A == (T)0) return false;
+  T sq = sqrt_v(disc);
+  T q = B > (T)0 ? -B - sq : -B + sq;
+  if (q == (T)0) {
+    tin = tout = (T)0;
+    return true;
+  }
+  T ra = q / A, rb = C / q;
+  tin = ra < rb ? ra : rb;
+  tout = ra < rb ? rb : ra;
+  return true;
+}
+
+// Convex polygon (counter-clockwise local vertices), Cyrus-Beck clipping of the line.
+template <typename T>
+__device__ bool chord_poly(const RayT<T>& r, const T* p, const T* verts, int first, int nv,
+                           T& tin, T& tout, int& ein, int& eout) {
+  T ox, oy, dx, dy;
+  to_local(r, p, ox, oy, dx, dy);
+  T t0 = -inf_v<T>(), t1 = inf_v<T>();
+  int e0 = -1, e1 = -1;
+  for (int i = 0; i < nv; i++) {
+    int j = (i + 1 == nv) ? 0 : i + 1;
+    T ax = verts[2 * (first + i)], ay = verts[2 * (first + i) + 1];
+    T bx = verts[2 * (first + j)], by = verts[2 * (first + j) + 1];
+    T nx = by - ay, ny = -(bx - ax);  // outward for CCW
+    T num = nx * (ax - ox) + ny * (ay - oy);
+    T den = nx * dx + ny * dy;
+    if (den == (T)0) {
+      if (num < (T)0) return false;
+      continue;
+    }
+    T t = num / den;
+    if (den < (T)0) {
+      if (t > t0) { t0 = t; e0 = i; }
+    } else {
+      if (t < t1) { t1 = t; e1 = i; }
+    }
+  }
+  if (!(t0 <= t1)) return false;
+  tin = t0; tout = t1; ein = e0; eout = e1;
+  return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// open primitives
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool hit_segment(const RayT<T>& r, const T* p, T& t_out) {
+  T ex = p[2], ey = p[3];
+  T den = r.dx * ey - r.dy * ex;
+  if (den == (T)0) return false;
+  T wx = p[0] - r.ox, wy = p[1] - r.oy;
+  T inv = rcp_v(den);
+  T t = (wx * ey - wy * ex) * inv;
+  T s = (wx * r.dy - wy * r.dx) * inv;
+  if (!(t > tmin_v<T>()) || s < (T)0 || s > (T)1) return false;
+  t_out = t;
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ T horner3(T A, T B, T C, T D, T s) {
+  return ((A * s + B) * s + C) * s + D;
+}
+
+// Nearest hit (smallest t > kTMin) of the ray with the cubic Bezier given in power basis
+// a = {a3x a3y a2x a2y a1x a1y a0x a0y}. [0,1] is split at the roots of the derivative of
+// f(s) = n . (B(s) - o) into monotone pieces and each piece with a sign change is
+// bisected (the oracle's rule). Returns false on a miss; sets `ambiguous` (FP32 only)
+// when a sign test at a piece end is too close to zero to trust.
+template <typename T>
+__device__ bool bezier_nearest(const RayT<T>& r, const T* a, T& t_best, T& s_best,
+                               bool& ambiguous) {
+  T nx = -r.dy, ny = r.dx;
+  T A = nx * a[0] + ny * a[1];
+  T B = nx * a[2] + ny * a[3];
+  T C = nx * a[4] + ny * a[5];
+  T D = nx * (a[6] - r.ox) + ny * (a[7] - r.oy);
+  T knots[4];
+  int nk = 0;
+  knots[nk++] = (T)0;
+  T qa = (T)3 * A, qb = (T)2 * B, qc = C;
+  T crit[2];
+  int nc = 0;
+  if (qa == (T)0) {
+    if (qb != (T)0) crit[nc++] = -qc / qb;
+  } else {
+    T disc = qb * qb - (T)4 * qa * qc;
+    if (disc >= (T)0) {
+      T sq = sqrt_v(disc);
+      T q = (T)(-0.5) * (qb + (qb >= (T)0 ? sq : -sq));
+      if (q != (T)0) {
+        crit[nc++] = q / qa;
+        crit[nc++] = qc / q;
+      } else {
+        crit[nc++] = (T)0;
+      }
+    }
+  }
+  if (nc == 2 && crit[0] > crit[1]) {
+    T tmp = crit[0];
+    crit[0] = crit[1];
+    crit[1] = tmp;
+  }
+  knots[1] = knots[2] = knots[3] = (T)1;
+  if (nc >= 1 && crit[0] > (T)0 && crit[0] < (T)1) {
+    knots[1] = crit[0];
+    nk = 2;
+  }
+  if (nc == 2 && crit[1] > (T)0 && crit[1] < (T)1 && crit[1] != (nk == 2 ? knots[1] : knots[0])) {
+    if (nk == 2) knots[2] = crit[1]; else knots[1] = crit[1];
+    nk++;
+  }
+  nk++;  // the end knot s = 1 (already stored)
+  T fk[4];
+  T scale = abs_v(A) + abs_v(B) + abs_v(C) + abs_v(D);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (i >= nk) break;
+    fk[i] = horner3(A, B, C, D, knots[i]);
+    if constexpr (sizeof(T) == 4) {
+      if (__builtin_fabsf(fk[i]) < kIsland * scale) ambiguous = true;
+    }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (ambiguous) return false;
+  }
+  bool found = false;
+  t_best = inf_v<T>();
+  constexpr int kMaxIt = sizeof(T) == 4 ? 48 : 200;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (i + 1 >= nk) break;
+    T l = knots[i], rr = knots[i + 1];
+    T fl = fk[i], fr = fk[i + 1];
+    T root;
+    if (fl == (T)0) {
+      root = l;
+    } else {
+      if (fr == (T)0) continue;
+      if ((fl < (T)0) == (fr < (T)0)) continue;
+      T lo = l, hi = rr, flo = fl;
+      for (int it = 0; it < kMaxIt; it++) {
+        T mid = (T)0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        T fm = horner3(A, B, C, D, mid);
+        if (fm == (T)0) {
+          lo = hi = mid;
+          break;
+        }
+        if ((fm < (T)0) == (flo < (T)0)) {
+          lo = mid;
+          flo = fm;
+        } else {
+          hi = mid;
+        }
+      }
+      root = (T)0.5 * (lo + hi);
+    }
+    T px = ((a[0] * root + a[2]) * root + a[4]) * root + a[6];
+    T py = ((a[1] * root + a[3]) * root + a[5]) * root + a[7];
+    T t = (px - r.ox) * r.dx + (py - r.oy) * r.dy;
+    if (t > tmin_v<T>() && t < t_best) {
+      t_best = t;
+      s_best = root;
+      found = true;
+    }
+  }
+  T f_last = nk == 2 ? fk[1] : (nk == 3 ? fk[2] : fk[3]);
+  if (f_last == (T)0) {
+    T px = a[0] + a[2] + a[4] + a[6], py = a[1] + a[3] + a[5] + a[7];
+    T t = (px - r.ox) * r.dx + (py - r.oy) * r.dy;
+    if (t > tmin_v<T>() && t < t_best) {
+      t_best = t;
+      s_best = (T)1;
+      found = true;
+    }
+  }
+  return found;
+}
+
+// ---------------------------------------------------------------------------------------
+// Chord of any closed primitive record (used for top-level shapes and Logic operands).
+// aux_in / aux_out: face or edge index of the chord ends (rect/polygon).
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool chord_any(const RayT<T>& r, const Rec<T>& R,
+                                          const Rec<double>* R64, const T* verts, T& tin,
+                                          T& tout, int& ain, int& aout) {
+  ain = aout = 0;
+  switch (R.kind) {
+    case REC_CIRCLE: return chord_circle(r, R.p, R64, tin, tout);
+    case REC_RECT: return chord_rect(r, R.p, tin, tout, ain, aout);
+    case REC_ELLIPSE: return chord_ellipse(r, R.p, R64, tin, tout);
+    case REC_POLY: return chord_poly(r, R.p, verts, R.aux, R.op, tin, tout, ain, aout);
+    default: return false;
+  }
+}
+
+// Nearest-hit state. code: bit0 = chord exit (else entry), bit1 = flip the normal,
+// bits 8.. = face/edge index. `rec` is the record whose surface was hit (a Logic operand
+// for boolean objects), `top` the object's first record. The runner-up from another
+// object (t2, top2) lets the FP32 pass hand near-ties to FP64.
+template <typename T>
+struct Nearest {
+  T t;
+  T s;        // Bezier parameter
+  int rec;
+  int top;
+  int code;
+  T t2;
+  int top2;
+};
+
+template <typename T>
+__device__ __forceinline__ void nearest_init(Nearest<T>& nb) {
+  nb.t = inf_v<T>();
+  nb.s = (T)0;
+  nb.rec = -1;
+  nb.top = -1;
+  nb.code = 0;
+  nb.t2 = inf_v<T>();
+  nb.top2 = -1;
+}
+
+// Candidates arrive in object order; strict < keeps the first minimum (tracer.rs:404,417).
+template <typename T>
+__device__ __forceinline__ void consider(Nearest<T>& nb, T t, int rec, int code, T s, int top) {
+  if (!(t > tmin_v<T>())) return;
+  if (t < nb.t) {
+    if (top != nb.top) {
+      nb.t2 = nb.t;
+      nb.top2 = nb.top;
+    }
+    nb.t = t;
+    nb.rec = rec;
+    nb.top = top;
+    nb.code = code;
+    nb.s = s;
+  } else if (t < nb.t2 && top != nb.top) {
+    nb.t2 = t;
+    nb.top2 = top;
+  }
+}
+
+// Boolean combination of two chords (closed convex operands). Candidate boundary points
+// are offered to `consider` in a-then-b order.
+template <typename T>
+__device__ void logic_candidates(Nearest<T>& nb, int op, bool hitA, T inA, T outA, int ainA,
+                                 int aoutA, int recA, bool hitB, T inB, T outB, int ainB,
+                                 int aoutB, int recB, int top) {
+  const int cA_in = (ainA << 8), cA_out = 1 | (aoutA << 8);
+  const int cB_in = (ainB << 8), cB_out = 1 | (aoutB << 8);
+  if (op == LG_LOGIC_AND) {
+    if (!hitA || !hitB) return;
+    // a-boundary points inside b, then b-boundary points inside a
+    if (inA > inB && inA < outB) consider(nb, inA, recA, cA_in, (T)0, top);
+    if (outA > inB && outA < outB) consider(nb, outA, recA, cA_out, (T)0, top);
+    if (inB > inA && inB < outA) consider(nb, inB, recB, cB_in, (T)0, top);
+    if (outB > inA && outB < outA) consider(nb, outB, recB, cB_out, (T)0, top);
+  } else if (op == LG_LOGIC_OR) {
+    // a-boundary points outside b, b-boundary points outside a
+    if (hitA) {
+      bool in_b_in = hitB && inA > inB && inA < outB;
+      bool in_b_out = hitB && outA > inB && outA < outB;
+      if (!in_b_in) consider(nb, inA, recA, cA_in, (T)0, top);
+      if (!in_b_out) consider(nb, outA, recA, cA_out, (T)0, top);
+    }
+    if (hitB) {
+      bool in_a_in = hitA && inB > inA && inB < outA;
+      bool in_a_out = hitA && outB > inA && outB < outA;
+      if (!in_a_in) consider(nb, inB, recB, cB_in, (T)0, top);
+      if (!in_a_out) consider(nb, outB, recB, cB_out, (T)0, top);
+    }
+  } else {
+    // a minus b: a-boundary outside b; b-boundary inside a with the normal flipped
+    if (hitA) {
+      bool in_b_in = hitB && inA > inB && inA < outB;
+      bool in_b_out = hitB && outA > inB && outA < outB;
+      if (!in_b_in) consider(nb, inA, recA, cA_in, (T)0, top);
+      if (!in_b_out) consider(nb, outA, recA, cA_out, (T)0, top);
+    }
+    if (hitB && hitA) {
+      if (inB > inA && inB < outA) consider(nb, inB, recB, cB_in | 2, (T)0, top);
+      if (outB > inA && outB < outA) consider(nb, outB, recB, cB_out | 2, (T)0, top);
+    }
+  }
+}
+
+// Tests one object (records starting at i) and returns the index of the next object.
+template <typename T>
+__device__ __forceinline__ int eval_object(const RayT<T>& r, const Rec<T>* recs,
+                                           const Rec<double>* recs64, const T* verts, int i,
+                                           Nearest<T>& nb) {
+  const Rec<T>& R = recs[i];
+  const int kind = R.kind;
+  if (kind == REC_SEG) {
+    T t;
+    if (hit_segment(r, R.p, t)) consider(nb, t, i, 0, (T)0, i);
+    return i + 1;
+  }
+  if (kind == REC_BEZ) {
+    T t, s = (T)0;
+    bool amb = false;
+    bool hit = bezier_nearest(r, R.p, t, s, amb);
+    if constexpr (sizeof(T) == 4) {
+      if (amb) {
+        RayT<double> rd{(double)r.ox, (double)r.oy, (double)r.dx, (double)r.dy};
+        double td, sd;
+        bool a2 = false;
+        hit = bezier_nearest<double>(rd, recs64[i].p, td, sd, a2);
+        t = (float)td;
+        s = (float)sd;
+      }
+    }
+    if (hit) consider(nb, t, i, 0, s, i);
+    return i + 1;
+  }
+  if (kind == REC_LOGIC) {
+    T inA = 0, outA = 0, inB = 0, outB = 0;
+    int ainA, aoutA, ainB, aoutB;
+    bool hA = chord_any(r, recs[i + 1], recs64 + i + 1, verts, inA, outA, ainA, aoutA);
+    bool hB = chord_any(r, recs[i + 2], recs64 + i + 2, verts, inB, outB, ainB, aoutB);
+    logic_candidates(nb, R.op, hA, inA, outA, ainA, aoutA, i + 1, hB, inB, outB, ainB, aoutB,
+                     i + 2, i);
+    return i + 3;
+  }
+  T tin, tout;
+  int ain, aout;
+  if (chord_any(r, R, recs64 + i, verts, tin, tout, ain, aout)) {
+    if (tin > tmin_v<T>())
+      consider(nb, tin, i, ain << 8, (T)0, i);
+    else
+      consider(nb, tout, i, 1 | (aout << 8), (T)0, i);
+  }
+  return i + 1;
+}
+
+// Walks every record for one ray.
+template <typename T>
+__device__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>* recs, const Rec<double>* recs64,
+                                  const T* verts, int n_recs) {
+  Nearest<T> nb;
+  nearest_init(nb);
+  int i = 0;
+  while (i < n_recs) i = eval_object(r, recs, recs64, verts, i, nb);
+  return nb;
+}
+
+// Outward (closed) or left (open) unit normal of the winning surface at p.
+template <typename T>
+__device__ void hit_normal(const Rec<T>& R, const T* verts, int code, T s, T px, T py, T& nx,
+                           T& ny) {
+  int face = code >> 8;
+  switch (R.kind) {
+    case REC_SEG: nx = R.p[4]; ny = R.p[5]; break;
+    case REC_BEZ: {
+      T tx = ((T)3 * R.p[0] * s + (T)2 * R.p[2]) * s + R.p[4];
+      T ty = ((T)3 * R.p[1] * s + (T)2 * R.p[3]) * s + R.p[5];
+      if (tx == (T)0 && ty == (T)0) {
+        // cusp: chord direction P3 - P0 = a3 + a2 + a1
+        tx = R.p[0] + R.p[2] + R.p[4];
+        ty = R.p[1] + R.p[3] + R.p[5];
+      }
+      nx = -ty;
+      ny = tx;
+      break;
+    }
+    case REC_CIRCLE: nx = px - R.p[0]; ny = py - R.p[1]; break;
+    case REC_RECT: {
+      T lx = (face == 1) ? (T)1 : (face == 3) ? (T)-1 : (T)0;
+      T ly = (face == 2) ? (T)1 : (face == 0) ? (T)-1 : (T)0;
+      nx = lx * R.p[2] + ly * R.p[4];
+      ny = lx * R.p[3] + ly * R.p[5];
+      break;
+    }
+    case REC_ELLIPSE: {
+      T lx = px - R.p[0], ly = py - R.p[1];
+      T x = lx * R.p[2] + ly * R.p[3], y = lx * R.p[4] + ly * R.p[5];
+      T gx = x / (R.p[6] * R.p[6]), gy = y / (R.p[7] * R.p[7]);
+      nx = gx * R.p[2] + gy * R.p[4];
+      ny = gx * R.p[3] + gy * R.p[5];
+      break;
+    }
+    case REC_POLY: {
+      int first = R.aux, nv = R.op;
+      int j = (face + 1 == nv) ? 0 : face + 1;
+      T ex = verts[2 * (first + j)] - verts[2 * (first + face)];
+      T ey = verts[2 * (first + j) + 1] - verts[2 * (first + face) + 1];
+      T lx = ey, ly = -ex;
+      nx = lx * R.p[2] + ly * R.p[4];
+      ny = lx * R.p[3] + ly * R.p[5];
+      break;
+    }
+    default: nx = (T)0; ny = (T)1; break;
+  }
+  normalize_v(nx, ny);
+  if (code & 2) {
+    nx = -nx;
+    ny = -ny;
+  }
+}
+
+// Ray::refract (tracer.rs:444-450): reflected and refracted directions and the
+// unpolarised Fresnel reflectance; false on total internal reflection.
+template <typename T>
+__device__ bool refract_dir(T dx, T dy, T nx, T ny, T n1, T n2, T& rx, T& ry, T& tx, T& ty,
+                            T& R) {
+  T ci = -(dx * nx + dy * ny);
+  if (ci < (T)0) {
+    nx = -nx;
+    ny = -ny;
+    ci = -ci;
+  }
+  rx = dx + (T)2 * ci * nx;
+  ry = dy + (T)2 * ci * ny;
+  normalize_v(rx, ry);
+  T eta = n1 / n2;
+  T k = (T)1 - eta * eta * ((T)1 - ci * ci);
+  if constexpr (sizeof(T) == 4) {
+    if (__builtin_fabsf(k) < kIsland) {
+      double rxd, ryd, txd, tyd, Rd;
+      bool ok = refract_dir<double>(dx, dy, nx, ny, n1, n2, rxd, ryd, txd, tyd, Rd);
+      rx = (float)rxd;
+      ry = (float)ryd;
+      tx = (float)txd;
+      ty = (float)tyd;
+      R = (float)Rd;
+      return ok;
+    }
+  }
+  if (k < (T)0) {
+    R = (T)1;
+    return false;
+  }
+  T ct = sqrt_v(k);
+  tx = eta * dx + (eta * ci - ct) * nx;
+  ty = eta * dy + (eta * ci - ct) * ny;
+  normalize_v(tx, ty);
+  T ds = n1 * ci + n2 * ct, dp = n2 * ci + n1 * ct;
+  T rs = ds != (T)0 ? (n1 * ci - n2 * ct) / ds : (T)1;
+  T rp = dp != (T)0 ? (n2 * ci - n1 * ct) / dp : (T)1;
+  R = (T)0.5 * (rs * rs + rp * rp);
+  return true;
+}
+
+// key = ray_index << kPathBits | path; a child appends one bit to the path
+// (0 reflected, 1 refracted).
+__device__ __forceinline__ uint64_t child_key(uint64_t key, uint64_t bit) {
+  const uint64_t mask = (1ull << kPathBits) - 1ull;
+  return (key & ~mask) | (((key & mask) << 1) & mask) | bit;
+}
+
+__device__ __forceinline__ bool below_cutoff(const float c[4], const float cut[4]) {
+  return (c[0] < cut[0] && c[1] < cut[1] && c[2] < cut[2]) || c[3] < cut[3];
+}
+
+// Light ray i of the scene (light.rs:103-115, 163-174, 225-249), generated in FP64.
+__device__ void gen_ray(const DevLight* lights, int n_lights, uint64_t gi, double& ox, double& oy,
+                        double& dx, double& dy, float c[4], double& ior) {
+  int l = 0;
+  while (l + 1 < n_lights && gi >= lights[l + 1].ray_base) l++;
+  const DevLight& L = lights[l];
+  uint64_t i = gi - L.ray_base;
+  double n = (double)L.num_rays;
+  if (L.kind == LG_LIGHT_POINT) {
+    double f = (double)i * M_PI * 2.0 / n;
+    ox = L.a[0];
+    oy = L.a[1];
+    dx = cos(f);
+    dy = sin(f);
+  } else if (L.kind == LG_LIGHT_SPOT) {
+    double angle = L.min_angle + ((double)(i + 1) / n) * L.spot_angle;
+    ox = L.a[0];
+    oy = L.a[1];
+    dx = L.sign * cos(angle);
+    dy = L.sign * sin(angle);
+  } else {
+    double r = (double)i / n;
+    ox = L.a[0] + r * L.e[0];
+    oy = L.a[1] + r * L.e[1];
+    dx = L.nrm[0];
+    dy = L.nrm[1];
+  }
+  c[0] = L.color[0];
+  c[1] = L.color[1];
+  c[2] = L.color[2];
+  c[3] = L.color[3];
+  ior = L.ior;
+}
+
+// Canvas exit (tracer.rs:484-486): nearest intersection with the canvas rect.
+template <typename T>
+__device__ bool canvas_exit(const RayT<T>& r, const double* cv, T& px, T& py) {
+  T t0 = -inf_v<T>(), t1 = inf_v<T>();
+  if (r.dx != (T)0) {
+    T ta = (cv[0] - r.ox) / r.dx, tb = (cv[1] - r.ox) / r.dx;
+    t0 = ta < tb ? ta : tb;
+    t1 = ta < tb ? tb : ta;
+  } else if (r.ox < cv[0] || r.ox > cv[1]) {
+    return false;
+  }
+  if (r.dy != (T)0) {
+    T ta = (cv[2] - r.oy) / r.dy, tb = (cv[3] - r.oy) / r.dy;
+    T lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
+    t0 = lo > t0 ? lo : t0;
+    t1 = hi < t1 ? hi : t1;
+  } else if (r.oy < cv[2] || r.oy > cv[3]) {
+    return false;
+  }
+  if (!(t0 <= t1)) return false;
+  T t;
+  if (t0 > tmin_v<T>())
+    t = t0;
+  else if (t1 > tmin_v<T>())
+    t = t1;
+  else
+    return false;
+  px = r.ox + t * r.dx;
+  py = r.oy + t * r.dy;
+  return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// One bounce generation.
+//
+// Ray state (origin, direction, refractive index) is FP64. The O(objects) culling loop runs
+// in T over the LDS records; for T = float the winning object (and a runner-up within the
+// tie tolerance) is re-evaluated in FP64, and the whole FP64 scan is redone when that
+// re-evaluation no longer clearly wins. Shading (normal, inside test, Snell/Fresnel, total
+// internal reflection, children) is FP64.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double tie_tol(double t) { return 1e-5 * (1.0 + t); }
+
+// FP64 answer for a ray whose FP32 culling pass found `nb` (kept out of line: it runs once
+// per ray, the culling loop once per object).
+__device__ __forceinline__ void refine_fp64(const RayT<double>& rd, const Nearest<float>& nb,
+                                         const Rec<double>* recs64, const double* verts64,
+                                         int n_recs, Nearest<double>& h) {
+  const bool tie = nb.top2 >= 0 && (double)nb.t2 <= (double)nb.t + tie_tol(nb.t);
+  if (tie) {
+    int a = nb.top < nb.top2 ? nb.top : nb.top2;
+    int b = nb.top < nb.top2 ? nb.top2 : nb.top;
+    eval_object<double>(rd, recs64, recs64, verts64, a, h);
+    eval_object<double>(rd, recs64, recs64, verts64, b, h);
+  } else {
+    eval_object<double>(rd, recs64, recs64, verts64, nb.top, h);
+  }
+  const bool unsure = h.top < 0 || (!tie && nb.top2 >= 0 && h.t > (double)nb.t2 - tie_tol(nb.t2));
+  if (unsure) h = nearest_hit<double>(rd, recs64, recs64, verts64, n_recs);
+}
+
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void trace_gen_kernel(TraceArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int kWaves = BLOCK / 64;
+  __shared__ uint32_t s_wseg[kWaves];
+  __shared__ uint32_t s_wch[kWaves];
+  __shared__ unsigned long long s_seg_base;
+  __shared__ uint32_t s_ch_base;
+
+  uint64_t count;
+  if (A.gen == 0) {
+    count = A.n_primary;
+  } else {
+    uint32_t c = *A.in_count;
+    count = c < A.q_cap ? c : A.q_cap;
+  }
+  if ((uint64_t)blockIdx.x * BLOCK >= count) return;
+
+  // scene -> LDS
+  Rec<T>* s_rec = reinterpret_cast<Rec<T>*>(smem);
+  uint32_t* s_obj_flags = reinterpret_cast<uint32_t*>(s_rec + A.n_recs);
+  int32_t* s_obj_rec = reinterpret_cast<int32_t*>(s_obj_flags + A.n_objs);
+  T* s_verts = reinterpret_cast<T*>(
+      (reinterpret_cast<uintptr_t>(s_obj_rec + A.n_objs) + 15) & ~uintptr_t(15));
+  {
+    const int4* src = reinterpret_cast<const int4*>(A.recs);
+    int4* dst = reinterpret_cast<int4*>(s_rec);
+    int n16 = A.n_recs * (int)(sizeof(Rec<T>) / 16);
+    for (int k = threadIdx.x; k < n16; k += BLOCK) dst[k] = src[k];
+    for (int k = threadIdx.x; k < A.n_objs; k += BLOCK) {
+      s_obj_flags[k] = A.obj_flags[k];
+      s_obj_rec[k] = A.obj_rec[k];
+    }
+    for (int k = threadIdx.x; k < 2 * A.n_verts; k += BLOCK) s_verts[k] = A.verts[k];
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < count; base += stride) {
+    uint64_t idx = base + threadIdx.x;
+    bool active = idx < count;
+    double ox = 0, oy = 0, dx = 1, dy = 0, ior = 1;
+    float c[4] = {0, 0, 0, 0};
+    uint64_t key = 0;
+    if (active) {
+      if (A.gen == 0) {
+        gen_ray(A.lights, A.n_lights, idx, ox, oy, dx, dy, c, ior);
+        key = (idx << kPathBits) | 1ull;
+        if (below_cutoff(c, A.cutoff)) active = false;
+      } else {
+        const QRay<double>& q = A.in_q[idx];
+        ox = q.o[0]; oy = q.o[1]; dx = q.d[0]; dy = q.d[1];
+        c[0] = q.c[0]; c[1] = q.c[1]; c[2] = q.c[2]; c[3] = q.c[3];
+        ior = q.ior;
+        key = q.key;
+      }
+    }
+
+    bool emit = false, keep0 = false, keep1 = false;
+    double px = 0, py = 0;
+    double d0x = 0, d0y = 0, d1x = 0, d1y = 0;
+    float c0[4] = {0, 0, 0, 0}, c1[4] = {0, 0, 0, 0};
+    double n_refr = ior;
+    if (active) {
+      const RayT<double> rd{ox, oy, dx, dy};
+      Nearest<double> h;
+      if constexpr (sizeof(T) == 8) {
+        h = nearest_hit<double>(rd, s_rec, A.recs64, s_verts, A.n_recs);
+      } else {
+        const RayT<float> rf{(float)ox, (float)oy, (float)dx, (float)dy};
+        Nearest<float> nb = nearest_hit<float>(rf, s_rec, A.recs64, s_verts, A.n_recs);
+        nearest_init(h);
+        if (nb.top >= 0) refine_fp64(rd, nb, A.recs64, A.verts64, A.n_recs, h);
+      }
+      if (h.rec >= 0) {
+        const Rec<double>& R = A.recs64[h.rec];
+        if (R.kind == REC_BEZ) {
+          double sp = h.s;
+          px = ((R.p[0] * sp + R.p[2]) * sp + R.p[4]) * sp + R.p[6];
+          py = ((R.p[1] * sp + R.p[3]) * sp + R.p[5]) * sp + R.p[7];
+        } else {
+          px = ox + h.t * dx;
+          py = oy + h.t * dy;
+        }
+        double nx, ny;
+        hit_normal<double>(R, A.verts64, h.code, h.s, px, py, nx, ny);
+        const int obj = R.obj;
+        const uint32_t fl = s_obj_flags[obj];
+        emit = true;
+        if (fl & OBJ_MATERIAL) {
+          const bool exiting = (fl & OBJ_CLOSED) && (dx * nx + dy * ny > 0.0);
+          double n2 = 1.0;
+          if (exiting) {
+            for (int o = 0; o < A.n_objs; o++) {
+              if (o == obj) continue;
+              if (!(s_obj_flags[o] & OBJ_MATERIAL)) continue;
+              if (contains_object<double>(A.recs64, A.verts64, s_obj_rec[o], px, py)) {
+                n2 = A.obj_n64[o];
+                break;
+              }
+            }
+          } else {
+            n2 = A.obj_n64[obj];
+          }
+          double Rf;
+          const bool has_refr = refract_dir<double>(dx, dy, nx, ny, ior, n2, d0x, d0y, d1x, d1y, Rf);
+          const float rf = (float)Rf;       // tracer.rs:454
+          const float omr = 1.0f - rf;      // tracer.rs:455
+          c0[0] = c[0] * rf; c0[1] = c[1] * rf; c0[2] = c[2] * rf; c0[3] = c[3];
+          keep0 = !below_cutoff(c0, A.cutoff);
+          if (has_refr) {
+            c1[0] = c[0] * omr; c1[1] = c[1] * omr; c1[2] = c[2] * omr; c1[3] = c[3];
+            keep1 = !below_cutoff(c1, A.cutoff);
+            n_refr = n2;
+          }
+        } else {
+          // Ray::reflect (tracer.rs:476-480)
+          const double dn = dx * nx + dy * ny;
+          d0x = dx - 2.0 * dn * nx;
+          d0y = dy - 2.0 * dn * ny;
+          normalize_v(d0x, d0y);
+          c0[0] = c[0]; c0[1] = c[1]; c0[2] = c[2]; c0[3] = c[3];
+          keep0 = true;  // same colour as the parent, which passed the cutoff
+        }
+      } else {
+        emit = canvas_exit<double>(rd, A.canvas, px, py);
+      }
+      if (A.last_gen) {
+        keep0 = false;
+        keep1 = false;
+      }
+    }
+
+    // ---- workgroup-aggregated append ----
+    const uint64_t m_seg = __ballot(emit);
+    const uint64_t m_c0 = __ballot(keep0);
+    const uint64_t m_c1 = __ballot(keep1);
+    const uint32_t pre_seg = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_seg >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m_seg, 0));
+    const uint32_t pre_c0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_c0 >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m_c0, 0));
+    const uint32_t pre_c1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_c1 >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m_c1, 0));
+    if (lane == 0) {
+      s_wseg[wave] = (uint32_t)__popcll(m_seg);
+      s_wch[wave] = (uint32_t)(__popcll(m_c0) + __popcll(m_c1));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t ts = 0, tc = 0;
+      for (int w = 0; w < kWaves; w++) {
+        uint32_t a = s_wseg[w], b = s_wch[w];
+        s_wseg[w] = ts;
+        s_wch[w] = tc;
+        ts += a;
+        tc += b;
+      }
+      s_seg_base = ts ? atomicAdd(A.seg_count, (unsigned long long)ts) : 0ull;
+      s_ch_base = tc ? atomicAdd(A.out_count, tc) : 0u;
+    }
+    __syncthreads();
+    if (emit) {
+      const uint64_t slot = s_seg_base + s_wseg[wave] + pre_seg;
+      if (slot < A.seg_cap) {
+        A.seg_pos[slot] = make_float4((float)ox, (float)oy, (float)px, (float)py);
+        A.seg_col[slot] = make_float4(c[0], c[1], c[2], c[3]);
+        A.seg_key[slot] = key;
+        if (A.seg_pos64) {
+          double* p64 = A.seg_pos64 + 4 * slot;
+          p64[0] = ox;
+          p64[1] = oy;
+          p64[2] = px;
+          p64[3] = py;
+        }
+      }
+    }
+    uint32_t ch_slot = s_ch_base + s_wch[wave] + pre_c0 + pre_c1;
+    if (keep0) {
+      if (ch_slot < A.q_cap) {
+        QRay<double>& q = A.out_q[ch_slot];
+        q.o[0] = px; q.o[1] = py; q.d[0] = d0x; q.d[1] = d0y;
+        q.c[0] = c0[0]; q.c[1] = c0[1]; q.c[2] = c0[2]; q.c[3] = c0[3];
+        q.ior = ior;
+        q.key = child_key(key, 0);
+      }
+      ch_slot++;
+    }
+    if (keep1) {
+      if (ch_slot < A.q_cap) {
+        QRay<double>& q = A.out_q[ch_slot];
+        q.o[0] = px; q.o[1] = py; q.d[0] = d1x; q.d[1] = d1y;
+        q.c[0] = c1[0]; q.c[1] = c1[1]; q.c[2] = c1[2]; q.c[3] = c1[3];
+        q.ior = n_refr;
+        q.key = child_key(key, 1);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------------------
+template <typename T>
+size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts) {
+  size_t b = (size_t)n_recs * sizeof(Rec<T>);
+  b += (size_t)n_objs * 8;
+  b = (b + 15) & ~size_t(15);
+  b += (size_t)n_verts * 2 * sizeof(T);
+  return b;
+}
+
+template <typename T>
+hipError_t launch_trace_gen(const TraceArgs<T>& args, int grid, hipStream_t stream) {
+  constexpr int kBlock = 256;
+  size_t lds = trace_lds_bytes<T>(args.n_recs, args.n_objs, args.n_verts);
+  hipLaunchKernelGGL((trace_gen_kernel<T, kBlock>), dim3(grid), dim3(kBlock), lds, stream, args);
+  return hipGetLastError();
+}
+
+template hipError_t launch_trace_gen<float>(const TraceArgs<float>&, int, hipStream_t);
+template hipError_t launch_trace_gen<double>(const TraceArgs<double>&, int, hipStream_t);
+template size_t trace_lds_bytes<float>(int, int, int);
+template size_t trace_lds_bytes<double>(int, int, int);
+
+}  // namespace lg

@@ -1,0 +1,443 @@
+"""Host mirror of the reference's scene/object/light API over the C ABI.
+
+Names and argument meaning follow src/light_garden/{object,light,tracer}.rs:
+  Object.new_mirror / new_curved_mirror / new_circle / new_rect / new_lens /
+  new_convex_polygon / new_ellipse / new_geo          (object.rs:57-113)
+  PointLight / SpotLight / DirectionalLight            (light.rs:78-250)
+  Tracer.push_object / push_light / replace_object / remove_object / remove_light /
+  clear / clear_objects / index_object / trace_all, fields max_bounce / cutoff_color /
+  chunk_size / canvas_bounds                           (tracer.rs:4-116, 276)
+The trace itself runs on the GPU (liblg.so); this module only builds lg_object / lg_light
+values and moves arrays.
+"""
+import ctypes as C
+import weakref
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from . import geo as G
+
+Color = Sequence[float]
+
+
+# ---- objects (object.rs) ----------------------------------------------------------------------
+@dataclass
+class Material:
+    refractive_index: float = 1.2  # Material::default (object.rs:441-446)
+
+
+@dataclass
+class Object:
+    geometry: object
+    material_opt: Optional[Material] = None
+    moved: bool = True
+
+    @staticmethod
+    def new_mirror(a, b) -> "Object":
+        return Object(G.LineSegment(tuple(a), tuple(b)), None)
+
+    @staticmethod
+    def new_curved_mirror(cubic: G.CubicBezier) -> "Object":
+        return Object(cubic, None)
+
+    @staticmethod
+    def new_circle(origin, radius) -> "Object":
+        return Object(G.Circle(tuple(origin), float(radius)), Material())
+
+    @staticmethod
+    def new_rect(origin, width, height) -> "Object":
+        return Object(G.Rect(tuple(origin), G.IDENTITY, float(width), float(height)), Material())
+
+    @staticmethod
+    def new_lens(origin, radius, distance) -> "Object":
+        return Object(G.lens(tuple(origin), float(radius), float(distance)), Material())
+
+    @staticmethod
+    def new_convex_polygon(points) -> "Object":
+        return Object(G.ConvexPolygon.new_convex_hull(points), Material())
+
+    @staticmethod
+    def new_ellipse(origin, a, b) -> "Object":
+        return Object(G.Ellipse(tuple(origin), float(a), float(b)), Material())
+
+    @staticmethod
+    def new_geo(geometry) -> "Object":
+        return Object(geometry, Material())
+
+    def get_material(self):
+        return self.material_opt
+
+    def get_origin(self):
+        return self.geometry.get_origin()
+
+    def set_origin(self, origin):
+        """HasOrigin::set_origin (object.rs:153-156): moves the shape so get_origin() == origin."""
+        ox, oy = self.get_origin()
+        self.geometry = self.geometry.translated(origin[0] - ox, origin[1] - oy)
+        self.moved = True
+
+    def to_abi(self) -> abi.LgObject:
+        o = abi.LgObject()
+        G.encode(self.geometry, o)
+        o.has_material = 1 if self.material_opt is not None else 0
+        o.refractive_index = self.material_opt.refractive_index if self.material_opt else 1.0
+        return o
+
+
+# ---- lights (light.rs) -------------------------------------------------------------------------
+@dataclass
+class PointLight:
+    position: Sequence[float]
+    num_rays: int
+    color: Color
+
+    def to_abi(self) -> abi.LgLight:
+        L = abi.LgLight()
+        L.kind = abi.LIGHT_POINT
+        L.num_rays = int(self.num_rays)
+        L.color[:] = [float(c) for c in self.color]
+        L.position[0], L.position[1] = self.position
+        return L
+
+    def get_origin(self):
+        return tuple(self.position)
+
+
+@dataclass
+class SpotLight:
+    position: Sequence[float]
+    spot_angle: float
+    spot_direction: Sequence[float]
+    num_rays: int
+    color: Color
+
+    def to_abi(self) -> abi.LgLight:
+        L = abi.LgLight()
+        L.kind = abi.LIGHT_SPOT
+        L.num_rays = int(self.num_rays)
+        L.color[:] = [float(c) for c in self.color]
+        L.position[0], L.position[1] = self.position
+        L.direction[0], L.direction[1] = self.spot_direction
+        L.spot_angle = float(self.spot_angle)
+        return L
+
+    def get_origin(self):
+        return tuple(self.position)
+
+
+@dataclass
+class DirectionalLight:
+    color: Color
+    num_rays: int
+    start: G.LineSegment
+
+    def to_abi(self) -> abi.LgLight:
+        L = abi.LgLight()
+        L.kind = abi.LIGHT_DIRECTIONAL
+        L.num_rays = int(self.num_rays)
+        L.color[:] = [float(c) for c in self.color]
+        L.position[0], L.position[1] = self.start.a
+        L.direction[0], L.direction[1] = self.start.b
+        return L
+
+    def get_origin(self):
+        return tuple(self.start.a)
+
+
+def objects_array(objs: Sequence[Object]):
+    arr = (abi.LgObject * max(1, len(objs)))()
+    for i, o in enumerate(objs):
+        arr[i] = o.to_abi()
+    return arr
+
+
+def lights_array(lights):
+    arr = (abi.LgLight * max(1, len(lights)))()
+    for i, l in enumerate(lights):
+        arr[i] = l.to_abi()
+    return arr
+
+
+# ---- device handles ----------------------------------------------------------------------------
+class Context:
+    """lg_ctx: one device, one HIP stream."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        self._children = weakref.WeakSet()
+        abi.call("lg_ctx_create", device, C.byref(self._h))
+
+    def _adopt(self, child):
+        self._children.add(child)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        abi.call("lg_ctx_synchronize", self._h)
+
+    def set_profiling(self, on: bool):
+        abi.call("lg_ctx_set_profiling", self._h, 1 if on else 0)
+
+    def stage_times(self):
+        arr = (abi.LgStageTime * 64)()
+        n = C.c_int()
+        abi.call("lg_ctx_stage_times", self._h, arr, 64, C.byref(n))
+        return {arr[i].name.decode(): (arr[i].ms, arr[i].launches, arr[i].work) for i in range(n.value)}
+
+    def close(self):
+        """Closes every live scene/buffer/image of this context, then the context."""
+        if self._h:
+            for child in list(self._children):
+                child.close()
+            abi.load().lg_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Segments:
+    """lg_segments: device SoA of segments (pos float4, color0 float4, [color1], key)."""
+
+    def __init__(self, ctx: Context, capacity: int = 1024, vertex_colors: bool = False):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        flags = abi.SEG_VERTEX_COLORS if vertex_colors else 0
+        abi.call("lg_segments_create", ctx.handle, int(capacity), flags, C.byref(self._h))
+        self.vertex_colors = vertex_colors
+        ctx._adopt(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def count(self) -> int:
+        n = C.c_uint64()
+        abi.call("lg_segments_count", self._h, C.byref(n))
+        return n.value
+
+    def upload(self, pos, color0, color1=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 4)
+        color0 = np.ascontiguousarray(color0, dtype=np.float32).reshape(-1, 4)
+        if color1 is not None:
+            color1 = np.ascontiguousarray(color1, dtype=np.float32).reshape(-1, 4)
+        n = pos.shape[0]
+        if color0.shape[0] != n or (color1 is not None and color1.shape[0] != n):
+            raise ValueError("pos/color arrays differ in length")
+        abi.call("lg_segments_upload", self._h, pos.ctypes.data, color0.ctypes.data,
+                 color1.ctypes.data if color1 is not None else None, n)
+
+    def download(self):
+        n = self.count()
+        pos = np.empty((n, 4), np.float32)
+        c0 = np.empty((n, 4), np.float32)
+        c1 = np.empty((n, 4), np.float32)
+        key = np.empty(n, np.uint64)
+        cnt = C.c_uint64()
+        abi.call("lg_segments_download", self._h, pos.ctypes.data, c0.ctypes.data,
+                 c1.ctypes.data, key.ctypes.data, n, C.byref(cnt))
+        return pos, c0, c1, key
+
+    def download_pos64(self):
+        n = self.count()
+        pos = np.empty((n, 4), np.float64)
+        cnt = C.c_uint64()
+        abi.call("lg_segments_download_pos64", self._h, pos.ctypes.data, n, C.byref(cnt))
+        return pos
+
+    def close(self):
+        if self._h:
+            abi.load().lg_segments_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Image:
+    """lg_image: FP32 RGBA accumulator plus the RGBA16F target."""
+
+    def __init__(self, ctx: Context, width: int, height: int, bounds=(-1.0, 1.0, -1.0, 1.0)):
+        self.ctx = ctx
+        self.width, self.height = int(width), int(height)
+        self.bounds = tuple(float(b) for b in bounds)
+        v = abi.LgView(self.bounds[0], self.bounds[1], self.bounds[2], self.bounds[3],
+                       self.width, self.height)
+        self._h = C.c_void_p()
+        abi.call("lg_image_create", ctx.handle, C.byref(v), C.byref(self._h))
+        ctx._adopt(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def view(self):
+        return abi.LgView(self.bounds[0], self.bounds[1], self.bounds[2], self.bounds[3],
+                          self.width, self.height)
+
+    def clear(self):
+        abi.call("lg_image_clear", self._h)
+
+    def accumulate(self, segs: Segments):
+        abi.call("lg_accumulate", self.ctx.handle, segs.handle, self._h)
+
+    def resolve(self):
+        abi.call("lg_resolve", self.ctx.handle, self._h)
+
+    def rgba16f(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.uint16)
+        abi.call("lg_image_download_rgba16f", self._h, out.ctypes.data, out.size)
+        return out
+
+    def rgba(self) -> np.ndarray:
+        """Decoded RGBA16F target as float32."""
+        return self.rgba16f().view(np.float16).astype(np.float32)
+
+    def accum(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.float32)
+        abi.call("lg_image_download_accum", self._h, out.ctypes.data, out.size)
+        return out
+
+    def close(self):
+        if self._h:
+            abi.load().lg_image_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- Tracer (tracer.rs) ------------------------------------------------------------------------
+class Tracer:
+    """The reference's Tracer: owns objects and lights; traces on the GPU."""
+
+    def __init__(self, ctx: Optional[Context] = None, canvas_bounds=(-1.0, 1.0, -1.0, 1.0)):
+        self.ctx = ctx or Context(0)
+        self.objects: List[Object] = []
+        self.lights: List[object] = []
+        self.max_bounce = 5                 # tracer.rs:37
+        self.cutoff_color = [0.001] * 4     # tracer.rs:38
+        self.chunk_size = 100               # tracer.rs:39 (rayon chunking; no GPU meaning)
+        self.canvas_bounds = tuple(canvas_bounds)  # left, right, bottom, top
+        self.fp64 = False
+        self._h = C.c_void_p()
+        abi.call("lg_scene_create", self.ctx.handle, C.byref(self._h))
+        self.ctx._adopt(self)
+        self._synced = False
+
+    # -- scene edits --
+    def push_object(self, obj: Object):
+        self.objects.append(obj)
+        self._synced = False
+
+    def push_light(self, light):
+        self.lights.append(light)
+        self._synced = False
+
+    def replace_object(self, ix: int, obj: Object):
+        self.objects[ix] = obj
+        self._synced = False
+
+    def remove_object(self, ix: int):
+        del self.objects[ix]
+        self._synced = False
+
+    def remove_light(self, ix: int):
+        del self.lights[ix]
+        self._synced = False
+
+    def index_object(self, ix: int) -> Object:
+        self._synced = False  # the caller may edit the returned object
+        return self.objects[ix]
+
+    def index_light(self, ix: int):
+        self._synced = False
+        return self.lights[ix]
+
+    def clear(self):
+        self.objects.clear()
+        self.lights.clear()
+        self._synced = False
+
+    def clear_objects(self):
+        self.objects.clear()
+        self._synced = False
+
+    def object_iterator(self):
+        return iter(self.objects)
+
+    def light_iterator(self):
+        return iter(self.lights)
+
+    # -- device --
+    def sync(self):
+        if self._synced:
+            return
+        objs = objects_array(self.objects)
+        lights = lights_array(self.lights)
+        abi.call("lg_scene_set_objects", self._h, objs, len(self.objects))
+        abi.call("lg_scene_set_lights", self._h, lights, len(self.lights))
+        self._synced = True
+
+    def params(self) -> abi.LgParams:
+        p = abi.LgParams()
+        p.max_bounce = int(self.max_bounce)
+        p.flags = abi.TRACE_FP64 if self.fp64 else 0
+        p.cutoff_color[:] = [float(c) for c in self.cutoff_color]
+        p.canvas[:] = list(self.canvas_bounds)
+        return p
+
+    def trace(self, segs: Optional[Segments] = None) -> Segments:
+        """Traces every light into device segments (unordered; keys give the order)."""
+        self.sync()
+        segs = segs or Segments(self.ctx)
+        p = self.params()
+        abi.call("lg_trace", self._h, C.byref(p), segs.handle)
+        return segs
+
+    def trace_all(self):
+        """Tracer::trace_all (tracer.rs:276): vertex pairs in the reference's order, as
+        (xy float64 (2S, 2), rgba float32 (2S, 4))."""
+        self.sync()
+        p = self.params()
+        n = C.c_uint64()
+        rc = abi.load().lg_trace_all(self._h, C.byref(p), None, None, 0, C.byref(n))
+        if rc not in (abi.LG_OK, abi.LG_E_CAPACITY):
+            abi.check(rc, "lg_trace_all")
+        xy = np.empty((n.value, 2), np.float64)
+        rgba = np.empty((n.value, 4), np.float32)
+        abi.call("lg_trace_all", self._h, C.byref(p), xy.ctypes.data, rgba.ctypes.data, n.value,
+                 C.byref(n))
+        return xy, rgba
+
+    def render(self, segs: Segments, img: Image):
+        """One frame: clear, trace, accumulate, resolve (LightGarden::draw + Renderer::render)."""
+        self.sync()
+        p = self.params()
+        abi.call("lg_render", self._h, C.byref(p), segs.handle, img.handle)
+
+    def close(self):
+        if self._h:
+            abi.load().lg_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

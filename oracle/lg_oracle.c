@@ -1,0 +1,1196 @@
+/*
+ * lg_oracle.c — FP64 CPU restatement of the light_garden trace and accumulation.
+ *
+ * TEST INFRASTRUCTURE ONLY (see lg_oracle.h): the checker for the HIP path, never part
+ * of the product.
+ *
+ * Reference files followed (paths relative to the reference root):
+ *   src/light_garden/tracer.rs   trace_all 276-330, trace 360-493
+ *   src/light_garden/light.rs    ray generation 103-115, 163-174, 225-249
+ *   src/light_garden/object.rs   shapes 6-120, contains 240-259, lens 387-434, material 436-447
+ *   src/light_garden/string_mod.rs  33-158
+ *   src/light_garden/mod.rs      blend 57-73
+ *   src/renderer.rs              projection 120-124, clear value 139-162/175
+ * The geometry crate collision2d (git rev a7b471b, not present) is restated from the
+ * mathematics; every choice it would have pinned is marked DEFINED.
+ */
+#define _GNU_SOURCE
+#include "lg_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------------------------ */
+/* small vector helpers                                                                  */
+/* ------------------------------------------------------------------------------------ */
+static inline double dot2(const double a[2], const double b[2]) { return a[0] * b[0] + a[1] * b[1]; }
+static inline double cross2(const double a[2], const double b[2]) { return a[0] * b[1] - a[1] * b[0]; }
+
+/* R column-major {m00, m10, m01, m11}: R*v and R^T*v. */
+static inline void rot_apply(const double r[4], const double v[2], double out[2]) {
+  double x = r[0] * v[0] + r[2] * v[1];
+  double y = r[1] * v[0] + r[3] * v[1];
+  out[0] = x;
+  out[1] = y;
+}
+static inline void rot_apply_t(const double r[4], const double v[2], double out[2]) {
+  double x = r[0] * v[0] + r[1] * v[1];
+  double y = r[2] * v[0] + r[3] * v[1];
+  out[0] = x;
+  out[1] = y;
+}
+static inline void normalize2(double v[2]) {
+  double l = sqrt(v[0] * v[0] + v[1] * v[1]);
+  if (l > 0) {
+    v[0] /= l;
+    v[1] /= l;
+  }
+}
+
+static int shape_is_framed(int kind) {
+  return kind == LG_SHAPE_CIRCLE || kind == LG_SHAPE_RECT || kind == LG_SHAPE_ELLIPSE ||
+         kind == LG_SHAPE_CONVEX_POLYGON || kind == LG_SHAPE_LOGIC;
+}
+
+static int push_hit(or_hit* hits, int k, int cap, double t, const double p[2], const double n[2]) {
+  if (k >= cap) return k;
+  hits[k].t = t;
+  hits[k].p[0] = p[0];
+  hits[k].p[1] = p[1];
+  hits[k].n[0] = n[0];
+  hits[k].n[1] = n[1];
+  return k + 1;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* primitive intersections, each in the shape's own frame                               */
+/* ------------------------------------------------------------------------------------ */
+
+/* Ray o + t d against the segment a-b. DEFINED: both endpoints count (0 <= s <= 1); the
+ * normal is the left normal of b - a. */
+static int isect_segment(const double a[2], const double b[2], const double o[2],
+                         const double d[2], or_hit* hits, int k, int cap) {
+  double e[2] = {b[0] - a[0], b[1] - a[1]};
+  double den = cross2(d, e);
+  if (den == 0.0) return k;
+  double w[2] = {a[0] - o[0], a[1] - o[1]};
+  double t = cross2(w, e) / den;
+  double s = cross2(w, d) / den;
+  if (!(t > OR_T_MIN) || s < 0.0 || s > 1.0) return k;
+  double p[2] = {o[0] + t * d[0], o[1] + t * d[1]};
+  double n[2] = {-e[1], e[0]};
+  normalize2(n);
+  return push_hit(hits, k, cap, t, p, n);
+}
+
+/* Circle centred at the frame origin. Outward normal. */
+static int isect_circle(double r, const double o[2], const double d[2], or_hit* hits, int k,
+                        int cap) {
+  double b = dot2(o, d);
+  double c = dot2(o, o) - r * r;
+  double disc = b * b - c;
+  if (disc < 0.0) return k;
+  double sq = sqrt(disc);
+  /* numerically stable pair of roots, ascending */
+  double q = (b > 0.0) ? (-b - sq) : (-b + sq);
+  double t1, t2;
+  if (q == 0.0) {
+    t1 = t2 = 0.0;
+  } else {
+    double ra = q, rb = c / q;
+    t1 = ra < rb ? ra : rb;
+    t2 = ra < rb ? rb : ra;
+  }
+  double ts[2] = {t1, t2};
+  for (int i = 0; i < 2; i++) {
+    double t = ts[i];
+    if (!(t > OR_T_MIN)) continue;
+    if (i == 1 && t2 == t1) continue;
+    double p[2] = {o[0] + t * d[0], o[1] + t * d[1]};
+    double n[2] = {p[0] / r, p[1] / r};
+    normalize2(n);
+    k = push_hit(hits, k, cap, t, p, n);
+  }
+  return k;
+}
+
+/* Axis-aligned rect [-w/2, w/2] x [-h/2, h/2] as its four edges (bottom, right, top,
+ * left), each with its outward normal. DEFINED: edge ends inclusive. */
+static int isect_rect(double w, double h, const double o[2], const double d[2], or_hit* hits,
+                      int k, int cap) {
+  double hw = 0.5 * w, hh = 0.5 * h;
+  /* bottom y = -hh, right x = hw, top y = hh, left x = -hw */
+  for (int e = 0; e < 4; e++) {
+    int axis = (e == 1 || e == 3) ? 0 : 1; /* coordinate fixed on this edge */
+    double val = (e == 0) ? -hh : (e == 1) ? hw : (e == 2) ? hh : -hw;
+    double lim = axis == 0 ? hh : hw;      /* half-extent along the edge */
+    if (d[axis] == 0.0) continue;
+    double t = (val - o[axis]) / d[axis];
+    if (!(t > OR_T_MIN)) continue;
+    double p[2] = {o[0] + t * d[0], o[1] + t * d[1]};
+    double along = p[1 - axis];
+    if (along < -lim || along > lim) continue;
+    double n[2] = {0.0, 0.0};
+    n[axis] = (e == 0 || e == 3) ? -1.0 : 1.0;
+    k = push_hit(hits, k, cap, t, p, n);
+  }
+  return k;
+}
+
+/* Ellipse (x/a)^2 + (y/b)^2 = 1; normal = normalised gradient. */
+static int isect_ellipse(double ea, double eb, const double o[2], const double d[2],
+                         or_hit* hits, int k, int cap) {
+  double os[2] = {o[0] / ea, o[1] / eb};
+  double ds[2] = {d[0] / ea, d[1] / eb};
+  double A = dot2(ds, ds), B = dot2(os, ds), C = dot2(os, os) - 1.0;
+  double disc = B * B - A * C;
+  if (disc < 0.0 || A == 0.0) return k;
+  double sq = sqrt(disc);
+  double q = (B > 0.0) ? (-B - sq) : (-B + sq);
+  double t1, t2;
+  if (q == 0.0) {
+    t1 = t2 = 0.0;
+  } else {
+    double ra = q / A, rb = C / q;
+    t1 = ra < rb ? ra : rb;
+    t2 = ra < rb ? rb : ra;
+  }
+  double ts[2] = {t1, t2};
+  for (int i = 0; i < 2; i++) {
+    double t = ts[i];
+    if (!(t > OR_T_MIN)) continue;
+    if (i == 1 && t2 == t1) continue;
+    double p[2] = {o[0] + t * d[0], o[1] + t * d[1]};
+    double n[2] = {p[0] / (ea * ea), p[1] / (eb * eb)};
+    normalize2(n);
+    k = push_hit(hits, k, cap, t, p, n);
+  }
+  return k;
+}
+
+/* Convex polygon, counter-clockwise vertices; outward (right) normal per edge. */
+static int isect_polygon(const double* v, int nv, const double o[2], const double d[2],
+                         or_hit* hits, int k, int cap) {
+  for (int i = 0; i < nv; i++) {
+    const double* a = v + 2 * i;
+    const double* b = v + 2 * ((i + 1) % nv);
+    double e[2] = {b[0] - a[0], b[1] - a[1]};
+    double den = cross2(d, e);
+    if (den == 0.0) continue;
+    double w[2] = {a[0] - o[0], a[1] - o[1]};
+    double t = cross2(w, e) / den;
+    double s = cross2(w, d) / den;
+    if (!(t > OR_T_MIN) || s < 0.0 || s > 1.0) continue;
+    double p[2] = {o[0] + t * d[0], o[1] + t * d[1]};
+    double n[2] = {e[1], -e[0]};
+    normalize2(n);
+    k = push_hit(hits, k, cap, t, p, n);
+  }
+  return k;
+}
+
+/* --- cubic Bezier ------------------------------------------------------------------ */
+static inline double horner3(double A, double B, double C, double D, double s) {
+  return ((A * s + B) * s + C) * s + D;
+}
+
+/* Real roots of A s^3 + B s^2 + C s + D in [0, 1], ascending. DEFINED: [0,1] is split at
+ * the critical points (roots of the derivative) into monotone pieces; a piece with a
+ * sign change is bisected to full double precision; exact zeros at piece ends count
+ * once. */
+static int cubic_roots_01(double A, double B, double C, double D, double roots[3]) {
+  double knots[4];
+  int nk = 0;
+  knots[nk++] = 0.0;
+  /* derivative 3A s^2 + 2B s + C */
+  double qa = 3.0 * A, qb = 2.0 * B, qc = C;
+  double crit[2];
+  int nc = 0;
+  if (qa == 0.0) {
+    if (qb != 0.0) crit[nc++] = -qc / qb;
+  } else {
+    double disc = qb * qb - 4.0 * qa * qc;
+    if (disc >= 0.0) {
+      double sq = sqrt(disc);
+      double q = -0.5 * (qb + (qb >= 0.0 ? sq : -sq));
+      if (q != 0.0) {
+        crit[nc++] = q / qa;
+        crit[nc++] = qc / q;
+      } else {
+        crit[nc++] = 0.0;
+      }
+    }
+  }
+  if (nc == 2 && crit[0] > crit[1]) {
+    double tmp = crit[0];
+    crit[0] = crit[1];
+    crit[1] = tmp;
+  }
+  for (int i = 0; i < nc; i++)
+    if (crit[i] > 0.0 && crit[i] < 1.0 && crit[i] != knots[nk - 1]) knots[nk++] = crit[i];
+  knots[nk++] = 1.0;
+
+  int nr = 0;
+  for (int i = 0; i + 1 < nk; i++) {
+    double l = knots[i], r = knots[i + 1];
+    double fl = horner3(A, B, C, D, l), fr = horner3(A, B, C, D, r);
+    if (fl == 0.0) {
+      if (nr == 0 || roots[nr - 1] != l) roots[nr++] = l;
+      continue;
+    }
+    if (fr == 0.0) continue; /* counted as the next piece's left end (or below) */
+    if ((fl < 0.0) == (fr < 0.0)) continue;
+    double lo = l, hi = r, flo = fl;
+    for (int it = 0; it < 200; it++) {
+      double mid = 0.5 * (lo + hi);
+      if (mid <= lo || mid >= hi) break;
+      double fm = horner3(A, B, C, D, mid);
+      if (fm == 0.0) {
+        lo = hi = mid;
+        break;
+      }
+      if ((fm < 0.0) == (flo < 0.0)) {
+        lo = mid;
+        flo = fm;
+      } else {
+        hi = mid;
+      }
+    }
+    roots[nr++] = 0.5 * (lo + hi);
+  }
+  if (horner3(A, B, C, D, 1.0) == 0.0 && (nr == 0 || roots[nr - 1] != 1.0)) roots[nr++] = 1.0;
+  return nr;
+}
+
+static int isect_bezier(const double* P, const double o[2], const double d[2], or_hit* hits,
+                        int k, int cap) {
+  /* power basis B(s) = a3 s^3 + a2 s^2 + a1 s + a0 */
+  double a3[2], a2[2], a1[2], a0[2];
+  for (int c = 0; c < 2; c++) {
+    double p0 = P[c], p1 = P[2 + c], p2 = P[4 + c], p3 = P[6 + c];
+    a3[c] = -p0 + 3.0 * p1 - 3.0 * p2 + p3;
+    a2[c] = 3.0 * p0 - 6.0 * p1 + 3.0 * p2;
+    a1[c] = 3.0 * (p1 - p0);
+    a0[c] = p0;
+  }
+  double nrm[2] = {-d[1], d[0]};
+  double rel[2] = {a0[0] - o[0], a0[1] - o[1]};
+  double A = dot2(nrm, a3), B = dot2(nrm, a2), C = dot2(nrm, a1), D = dot2(nrm, rel);
+  double roots[3];
+  int nr = cubic_roots_01(A, B, C, D, roots);
+  for (int i = 0; i < nr; i++) {
+    double s = roots[i];
+    double p[2];
+    for (int c = 0; c < 2; c++) p[c] = ((a3[c] * s + a2[c]) * s + a1[c]) * s + a0[c];
+    double t = (p[0] - o[0]) * d[0] + (p[1] - o[1]) * d[1];
+    if (!(t > OR_T_MIN)) continue;
+    double tg[2];
+    for (int c = 0; c < 2; c++) tg[c] = (3.0 * a3[c] * s + 2.0 * a2[c]) * s + a1[c];
+    if (tg[0] == 0.0 && tg[1] == 0.0) {
+      tg[0] = P[6] - P[0];
+      tg[1] = P[7] - P[1];
+    }
+    double n[2] = {-tg[1], tg[0]};
+    normalize2(n);
+    k = push_hit(hits, k, cap, t, p, n);
+  }
+  return k;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* shape tree: intersection and containment                                             */
+/* ------------------------------------------------------------------------------------ */
+static int contains_node(const lg_object* obj, int ni, const double p[2]);
+
+/* o, d in the parent frame of node ni; hits returned in the parent frame. */
+static int isect_node(const lg_object* obj, int ni, const double o[2], const double d[2],
+                      or_hit* hits, int cap) {
+  const lg_shape* s = &obj->shapes[ni];
+  if (s->kind == LG_SHAPE_LINE_SEGMENT) return isect_segment(s->p, s->p + 2, o, d, hits, 0, cap);
+  if (s->kind == LG_SHAPE_CUBIC_BEZIER) return isect_bezier(s->p, o, d, hits, 0, cap);
+  if (!shape_is_framed(s->kind)) return 0;
+
+  double rel[2] = {o[0] - s->origin[0], o[1] - s->origin[1]};
+  double lo[2], ld[2];
+  rot_apply_t(s->rotation, rel, lo);
+  rot_apply_t(s->rotation, d, ld);
+  int k = 0;
+  switch (s->kind) {
+    case LG_SHAPE_CIRCLE: k = isect_circle(s->p[0], lo, ld, hits, 0, cap); break;
+    case LG_SHAPE_RECT: k = isect_rect(s->p[0], s->p[1], lo, ld, hits, 0, cap); break;
+    case LG_SHAPE_ELLIPSE: k = isect_ellipse(s->p[0], s->p[1], lo, ld, hits, 0, cap); break;
+    case LG_SHAPE_CONVEX_POLYGON:
+      k = isect_polygon(obj->vertices + 2 * s->vertex_offset, s->n_vertices, lo, ld, hits, 0, cap);
+      break;
+    case LG_SHAPE_LOGIC: {
+      /* DEFINED boolean rule: And keeps a-hits inside b and b-hits inside a; Or keeps
+       * hits outside the other operand; AndNot keeps a-hits outside b and b-hits inside a
+       * with the b normal flipped. Hits are listed a first, then b. */
+      or_hit ha[OR_MAX_HITS], hb[OR_MAX_HITS];
+      int na = isect_node(obj, s->child_a, lo, ld, ha, OR_MAX_HITS);
+      int nb = isect_node(obj, s->child_b, lo, ld, hb, OR_MAX_HITS);
+      for (int i = 0; i < na; i++) {
+        int inb = contains_node(obj, s->child_b, ha[i].p);
+        int keep = (s->op == LG_LOGIC_AND) ? inb : !inb;
+        if (keep) k = push_hit(hits, k, cap, ha[i].t, ha[i].p, ha[i].n);
+      }
+      for (int i = 0; i < nb; i++) {
+        int ina = contains_node(obj, s->child_a, hb[i].p);
+        int keep = (s->op == LG_LOGIC_OR) ? !ina : ina;
+        if (!keep) continue;
+        double n[2] = {hb[i].n[0], hb[i].n[1]};
+        if (s->op == LG_LOGIC_AND_NOT) {
+          n[0] = -n[0];
+          n[1] = -n[1];
+        }
+        k = push_hit(hits, k, cap, hb[i].t, hb[i].p, n);
+      }
+      break;
+    }
+    default: break;
+  }
+  /* back to the parent frame; t is unchanged because R is a rotation */
+  for (int i = 0; i < k; i++) {
+    double pw[2], nw[2];
+    rot_apply(s->rotation, hits[i].p, pw);
+    rot_apply(s->rotation, hits[i].n, nw);
+    hits[i].p[0] = pw[0] + s->origin[0];
+    hits[i].p[1] = pw[1] + s->origin[1];
+    hits[i].n[0] = nw[0];
+    hits[i].n[1] = nw[1];
+  }
+  return k;
+}
+
+/* DEFINED: containment is strict (boundary points are outside); line segments and Bezier
+ * curves contain nothing (object.rs:243-244). */
+static int contains_node(const lg_object* obj, int ni, const double p[2]) {
+  const lg_shape* s = &obj->shapes[ni];
+  if (!shape_is_framed(s->kind)) return 0;
+  double rel[2] = {p[0] - s->origin[0], p[1] - s->origin[1]};
+  double q[2];
+  rot_apply_t(s->rotation, rel, q);
+  switch (s->kind) {
+    case LG_SHAPE_CIRCLE: return dot2(q, q) < s->p[0] * s->p[0];
+    case LG_SHAPE_RECT: return fabs(q[0]) < 0.5 * s->p[0] && fabs(q[1]) < 0.5 * s->p[1];
+    case LG_SHAPE_ELLIPSE: {
+      double x = q[0] / s->p[0], y = q[1] / s->p[1];
+      return x * x + y * y < 1.0;
+    }
+    case LG_SHAPE_CONVEX_POLYGON: {
+      const double* v = obj->vertices + 2 * s->vertex_offset;
+      int nv = s->n_vertices;
+      if (nv < 3) return 0;
+      for (int i = 0; i < nv; i++) {
+        const double* a = v + 2 * i;
+        const double* b = v + 2 * ((i + 1) % nv);
+        double e[2] = {b[0] - a[0], b[1] - a[1]};
+        double w[2] = {q[0] - a[0], q[1] - a[1]};
+        if (!(cross2(e, w) > 0.0)) return 0;
+      }
+      return 1;
+    }
+    case LG_SHAPE_LOGIC: {
+      int ca = contains_node(obj, s->child_a, q);
+      int cb = contains_node(obj, s->child_b, q);
+      if (s->op == LG_LOGIC_AND) return ca && cb;
+      if (s->op == LG_LOGIC_OR) return ca || cb;
+      return ca && !cb;
+    }
+    default: return 0;
+  }
+}
+
+int or_intersect_object(const lg_object* obj, const double o[2], const double d[2], or_hit* hits,
+                        int cap) {
+  if (obj->n_shapes <= 0) return 0;
+  return isect_node(obj, 0, o, d, hits, cap);
+}
+
+int or_contains_object(const lg_object* obj, const double p[2]) {
+  if (obj->n_shapes <= 0) return 0;
+  return contains_node(obj, 0, p);
+}
+
+int or_object_is_closed(const lg_object* obj) {
+  if (obj->n_shapes <= 0) return 0;
+  return shape_is_framed(obj->shapes[0].kind);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Ray::refract / Ray::reflect (called at tracer.rs:444-450, 476-480)                    */
+/* ------------------------------------------------------------------------------------ */
+/* DEFINED: the normal is flipped to face the incoming ray; directions are unit; the
+ * reflectance is the unpolarised Fresnel average (rs^2 + rp^2) / 2; total internal
+ * reflection gives no refracted ray and reflectance 1. */
+int or_refract(const double d[2], const double n_in[2], double n1, double n2, double refl[2],
+               double refr[2], double* reflectance) {
+  double n[2] = {n_in[0], n_in[1]};
+  double ci = -dot2(d, n);
+  if (ci < 0.0) {
+    n[0] = -n[0];
+    n[1] = -n[1];
+    ci = -ci;
+  }
+  refl[0] = d[0] + 2.0 * ci * n[0];
+  refl[1] = d[1] + 2.0 * ci * n[1];
+  normalize2(refl);
+  double eta = n1 / n2;
+  double k = 1.0 - eta * eta * (1.0 - ci * ci);
+  if (k < 0.0) {
+    *reflectance = 1.0;
+    return 0;
+  }
+  double ct = sqrt(k);
+  refr[0] = eta * d[0] + (eta * ci - ct) * n[0];
+  refr[1] = eta * d[1] + (eta * ci - ct) * n[1];
+  normalize2(refr);
+  double ds = n1 * ci + n2 * ct, dp = n2 * ci + n1 * ct;
+  double rs = ds != 0.0 ? (n1 * ci - n2 * ct) / ds : 1.0;
+  double rp = dp != 0.0 ? (n2 * ci - n1 * ct) / dp : 1.0;
+  *reflectance = 0.5 * (rs * rs + rp * rp);
+  return 1;
+}
+
+void or_reflect(const double d[2], const double n[2], double out[2]) {
+  double dn = dot2(d, n);
+  out[0] = d[0] - 2.0 * dn * n[0];
+  out[1] = d[1] - 2.0 * dn * n[1];
+  normalize2(out);
+}
+
+/* Canvas exit (tracer.rs:484-486): the canvas Rect {left,right,bottom,top}. DEFINED:
+ * get_first() is the nearest intersection. */
+int or_canvas_exit(const double canvas[4], const double o[2], const double d[2], double p[2]) {
+  double c[2] = {0.5 * (canvas[0] + canvas[1]), 0.5 * (canvas[2] + canvas[3])};
+  double w = canvas[1] - canvas[0], h = canvas[3] - canvas[2];
+  double lo[2] = {o[0] - c[0], o[1] - c[1]};
+  or_hit hits[4];
+  int k = isect_rect(w, h, lo, d, hits, 0, 4);
+  if (k == 0) return 0;
+  int best = 0;
+  for (int i = 1; i < k; i++)
+    if (hits[i].t < hits[best].t) best = i;
+  p[0] = hits[best].p[0] + c[0];
+  p[1] = hits[best].p[1] + c[1];
+  return 1;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* lights (light.rs)                                                                     */
+/* ------------------------------------------------------------------------------------ */
+void or_light_ray(const lg_light* L, uint64_t i, double o[2], double d[2]) {
+  double n = (double)L->num_rays;
+  if (L->kind == LG_LIGHT_POINT) {
+    /* light.rs:168-173: f = i * PI * 2 / n, dir = (cos f, sin f) */
+    double f = (double)i * M_PI * 2.0 / n;
+    o[0] = L->position[0];
+    o[1] = L->position[1];
+    d[0] = cos(f);
+    d[1] = sin(f);
+  } else if (L->kind == LG_LIGHT_SPOT) {
+    /* light.rs:230-248 */
+    double dx = L->direction[0], dy = L->direction[1];
+    double dir_angle;
+    if (fabs(dx) < OR_EPSILON)
+      dir_angle = dy >= 0.0 ? M_PI * 0.5 : -M_PI * 0.5;
+    else
+      dir_angle = atan(dy / dx);
+    double min_angle = dir_angle - 0.5 * L->spot_angle;
+    double step = (double)(i + 1);
+    double angle = min_angle + (step / n) * L->spot_angle;
+    double sign = signbit(dx) ? -1.0 : 1.0; /* f64::signum */
+    o[0] = L->position[0];
+    o[1] = L->position[1];
+    d[0] = sign * cos(angle);
+    d[1] = sign * sin(angle);
+  } else {
+    /* light.rs:108-114 with LineSegment a->b. DEFINED: eval_at_r(-i/n) = a + (i/n)(b - a)
+     * and get_normal() = left normal of b - a. */
+    double a[2] = {L->position[0], L->position[1]};
+    double e[2] = {L->direction[0] - a[0], L->direction[1] - a[1]};
+    double r = (double)i / n;
+    o[0] = a[0] + r * e[0];
+    o[1] = a[1] + r * e[1];
+    d[0] = -e[1];
+    d[1] = e[0];
+    normalize2(d);
+  }
+}
+
+/* tracer.rs:280-287: n of the last object with a material containing p, else 1. */
+double or_initial_ior(const lg_object* objs, uint32_t n_obj, const double p[2]) {
+  double n = 1.0;
+  for (uint32_t i = 0; i < n_obj; i++)
+    if (or_contains_object(&objs[i], p) && objs[i].has_material) n = objs[i].refractive_index;
+  return n;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* segment list                                                                          */
+/* ------------------------------------------------------------------------------------ */
+static int segs_reserve(or_segs* s, uint64_t need) {
+  if (need <= s->cap) return 1;
+  uint64_t cap = s->cap ? s->cap : 256;
+  while (cap < need) cap *= 2;
+  double* pos = (double*)realloc(s->pos, cap * 4 * sizeof(double));
+  if (!pos) return 0;
+  s->pos = pos;
+  float* col = (float*)realloc(s->color, cap * 4 * sizeof(float));
+  if (!col) return 0;
+  s->color = col;
+  uint64_t* key = (uint64_t*)realloc(s->key, cap * sizeof(uint64_t));
+  if (!key) return 0;
+  s->key = key;
+  s->cap = cap;
+  return 1;
+}
+
+static int segs_push(or_segs* s, const double a[2], const double b[2], const float c[4],
+                     uint64_t key) {
+  if (!segs_reserve(s, s->n + 1)) return 0;
+  double* p = s->pos + 4 * s->n;
+  p[0] = a[0];
+  p[1] = a[1];
+  p[2] = b[0];
+  p[3] = b[1];
+  memcpy(s->color + 4 * s->n, c, 4 * sizeof(float));
+  s->key[s->n] = key;
+  s->n++;
+  return 1;
+}
+
+uint64_t or_segs_count(const or_segs* s) { return s ? s->n : 0; }
+
+void or_segs_copy(const or_segs* s, double* pos, float* color, float* color1, uint64_t* key) {
+  if (pos) memcpy(pos, s->pos, s->n * 4 * sizeof(double));
+  if (color) memcpy(color, s->color, s->n * 4 * sizeof(float));
+  if (color1) memcpy(color1, s->color1 ? s->color1 : s->color, s->n * 4 * sizeof(float));
+  if (key) memcpy(key, s->key, s->n * sizeof(uint64_t));
+}
+
+void or_segs_free(or_segs* s) {
+  if (!s) return;
+  free(s->pos);
+  free(s->color);
+  free(s->color1);
+  free(s->key);
+  free(s);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* the trace (tracer.rs:360-493)                                                         */
+/* ------------------------------------------------------------------------------------ */
+typedef struct tray {
+  double o[2], d[2];
+  float c[4];
+  double n;
+  uint64_t path;
+} tray;
+
+typedef struct trace_scene {
+  const lg_object* objs;
+  uint32_t n_obj;
+  const lg_params* prm;
+} trace_scene;
+
+typedef struct ray_vec {
+  tray* v;
+  size_t n, cap;
+} ray_vec;
+
+static int rv_push(ray_vec* r, const tray* t) {
+  if (r->n == r->cap) {
+    size_t cap = r->cap ? 2 * r->cap : 64;
+    tray* v = (tray*)realloc(r->v, cap * sizeof(tray));
+    if (!v) return 0;
+    r->v = v;
+    r->cap = cap;
+  }
+  r->v[r->n++] = *t;
+  return 1;
+}
+
+/* One primary ray's breadth-first bounce tree. Segment key = ray_index << 24 | path, with
+ * path = 1 for the primary ray and path' = path << 1 | (refracted ? 1 : 0) per child, so
+ * sorting by key gives the reference's emission order. */
+static int trace_one(const trace_scene* S, const double o[2], const double d[2],
+                     const float color[4], double ior, uint64_t ray_index, or_segs* out,
+                     ray_vec* cur, ray_vec* next) {
+  const lg_params* P = S->prm;
+  cur->n = 0;
+  next->n = 0;
+  tray r0;
+  r0.o[0] = o[0];
+  r0.o[1] = o[1];
+  r0.d[0] = d[0];
+  r0.d[1] = d[1];
+  memcpy(r0.c, color, sizeof(r0.c));
+  r0.n = ior;
+  r0.path = 1;
+  if (!rv_push(cur, &r0)) return 0;
+  or_hit hits[OR_MAX_HITS];
+  for (uint32_t gen = 0; gen < P->max_bounce; gen++) {
+    if (cur->n == 0) break;
+    for (size_t ri = 0; ri < cur->n; ri++) {
+      const tray* ray = &cur->v[ri];
+      const float* c = ray->c;
+      const float* cut = P->cutoff_color;
+      /* tracer.rs:378-384 */
+      if ((c[0] < cut[0] && c[1] < cut[1] && c[2] < cut[2]) || c[3] < cut[3]) continue;
+      /* nearest hit over all objects, strict < in index order (tracer.rs:413-423) */
+      double nearest = 1.79769313486231570815e+308;
+      int best_obj = -1;
+      double bp[2] = {0, 0}, bn[2] = {0, 0};
+      for (uint32_t oi = 0; oi < S->n_obj; oi++) {
+        int k = or_intersect_object(&S->objs[oi], ray->o, ray->d, hits, OR_MAX_HITS);
+        for (int h = 0; h < k; h++) {
+          double dx = hits[h].p[0] - ray->o[0], dy = hits[h].p[1] - ray->o[1];
+          double dsq = dx * dx + dy * dy;
+          if (dsq < nearest) {
+            nearest = dsq;
+            best_obj = (int)oi;
+            bp[0] = hits[h].p[0];
+            bp[1] = hits[h].p[1];
+            bn[0] = hits[h].n[0];
+            bn[1] = hits[h].n[1];
+          }
+        }
+      }
+      uint64_t key = (ray_index << 24) | ray->path;
+      if (best_obj >= 0) {
+        const lg_object* obj = &S->objs[best_obj];
+        if (obj->has_material) {
+          /* tracer.rs:430-442. The reference asks obj.contains(ray.origin). DEFINED: for a
+           * closed shape this is the crossing direction at the hit (the ray leaves the
+           * shape iff d . n_out > 0), which equals contains(origin) for every origin off
+           * the boundary and settles the boundary origins of secondary rays. Open shapes
+           * contain nothing. */
+          int exiting = or_object_is_closed(obj) && dot2(ray->d, bn) > 0.0;
+          double n2 = 1.0;
+          if (exiting) {
+            for (uint32_t oi = 0; oi < S->n_obj; oi++) {
+              if ((int)oi == best_obj) continue;
+              if (or_contains_object(&S->objs[oi], bp) && S->objs[oi].has_material) {
+                n2 = S->objs[oi].refractive_index;
+                break;
+              }
+            }
+          } else {
+            n2 = obj->refractive_index;
+          }
+          double refl[2], refr[2], R;
+          int has_refr = or_refract(ray->d, bn, ray->n, n2, refl, refr, &R);
+          if (!segs_push(out, ray->o, bp, c, key)) return 0;
+          float rf = (float)R;       /* tracer.rs:454 */
+          float omr = 1.0f - rf;     /* tracer.rs:455 */
+          tray child;
+          child.o[0] = bp[0];
+          child.o[1] = bp[1];
+          child.d[0] = refl[0];
+          child.d[1] = refl[1];
+          child.c[0] = c[0] * rf;
+          child.c[1] = c[1] * rf;
+          child.c[2] = c[2] * rf;
+          child.c[3] = c[3];
+          child.n = ray->n;
+          child.path = ray->path << 1;
+          if (!rv_push(next, &child)) return 0;
+          if (has_refr) {
+            child.d[0] = refr[0];
+            child.d[1] = refr[1];
+            child.c[0] = c[0] * omr;
+            child.c[1] = c[1] * omr;
+            child.c[2] = c[2] * omr;
+            child.c[3] = c[3];
+            child.n = n2;
+            child.path = (ray->path << 1) | 1u;
+            if (!rv_push(next, &child)) return 0;
+          }
+        } else {
+          /* mirror, tracer.rs:473-481 */
+          if (!segs_push(out, ray->o, bp, c, key)) return 0;
+          tray child;
+          child.o[0] = bp[0];
+          child.o[1] = bp[1];
+          or_reflect(ray->d, bn, child.d);
+          memcpy(child.c, c, sizeof(child.c));
+          child.n = ray->n;
+          child.path = ray->path << 1;
+          if (!rv_push(next, &child)) return 0;
+        }
+      } else {
+        /* tracer.rs:482-488 */
+        double pe[2];
+        if (or_canvas_exit(P->canvas, ray->o, ray->d, pe))
+          if (!segs_push(out, ray->o, pe, c, key)) return 0;
+      }
+    }
+    ray_vec tmp = *cur;
+    *cur = *next;
+    *next = tmp;
+    next->n = 0;
+  }
+  return 1;
+}
+
+/* work list: (light, first ray, ray count) chunks */
+typedef struct chunk {
+  uint32_t light;
+  uint64_t begin, end; /* ray index range within the light */
+} chunk;
+
+typedef struct trace_job {
+  trace_scene S;
+  const lg_light* lights;
+  const double* light_ior;
+  const uint64_t* light_base;
+  const chunk* chunks;
+  size_t n_chunks;
+  uint64_t stride;
+  or_segs* chunk_out; /* one list per chunk */
+  size_t next_chunk;  /* guarded by mu */
+  pthread_mutex_t mu;
+  int failed;
+} trace_job;
+
+static void* trace_worker(void* arg) {
+  trace_job* J = (trace_job*)arg;
+  ray_vec cur = {0}, next = {0};
+  for (;;) {
+    pthread_mutex_lock(&J->mu);
+    size_t ci = J->next_chunk++;
+    pthread_mutex_unlock(&J->mu);
+    if (ci >= J->n_chunks) break;
+    const chunk* C = &J->chunks[ci];
+    const lg_light* L = &J->lights[C->light];
+    for (uint64_t i = C->begin; i < C->end; i++) {
+      uint64_t g = J->light_base[C->light] + i;
+      if (J->stride > 1 && (g % J->stride) != 0) continue;
+      double o[2], d[2];
+      or_light_ray(L, i, o, d);
+      if (!trace_one(&J->S, o, d, L->color, J->light_ior[C->light], g, &J->chunk_out[ci], &cur,
+                     &next)) {
+        J->failed = 1;
+        break;
+      }
+    }
+  }
+  free(cur.v);
+  free(next.v);
+  return NULL;
+}
+
+or_segs* or_trace(const lg_object* objs, uint32_t n_obj, const lg_light* lights,
+                  uint32_t n_lights, const lg_params* params, uint64_t ray_stride,
+                  int n_threads, int chunk_size) {
+  if (n_threads < 1) n_threads = 1;
+  if (chunk_size < 1) chunk_size = 100;
+  if (ray_stride < 1) ray_stride = 1;
+  double* light_ior = (double*)calloc(n_lights + 1, sizeof(double));
+  uint64_t* light_base = (uint64_t*)calloc(n_lights + 1, sizeof(uint64_t));
+  size_t n_chunks = 0;
+  uint64_t base = 0;
+  for (uint32_t l = 0; l < n_lights; l++) {
+    light_base[l] = base;
+    base += lights[l].num_rays;
+    light_ior[l] = or_initial_ior(objs, n_obj, lights[l].position);
+    n_chunks += (lights[l].num_rays + (uint64_t)chunk_size - 1) / (uint64_t)chunk_size;
+  }
+  chunk* chunks = (chunk*)calloc(n_chunks + 1, sizeof(chunk));
+  size_t c = 0;
+  for (uint32_t l = 0; l < n_lights; l++)
+    for (uint64_t b = 0; b < lights[l].num_rays; b += (uint64_t)chunk_size) {
+      chunks[c].light = l;
+      chunks[c].begin = b;
+      uint64_t e = b + (uint64_t)chunk_size;
+      chunks[c].end = e < lights[l].num_rays ? e : lights[l].num_rays;
+      c++;
+    }
+  trace_job J;
+  memset(&J, 0, sizeof(J));
+  J.S.objs = objs;
+  J.S.n_obj = n_obj;
+  J.S.prm = params;
+  J.lights = lights;
+  J.light_ior = light_ior;
+  J.light_base = light_base;
+  J.chunks = chunks;
+  J.n_chunks = n_chunks;
+  J.stride = ray_stride;
+  J.chunk_out = (or_segs*)calloc(n_chunks + 1, sizeof(or_segs));
+  pthread_mutex_init(&J.mu, NULL);
+  if (n_threads == 1) {
+    trace_worker(&J);
+  } else {
+    pthread_t* th = (pthread_t*)calloc((size_t)n_threads, sizeof(pthread_t));
+    for (int t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, trace_worker, &J);
+    for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
+    free(th);
+  }
+  pthread_mutex_destroy(&J.mu);
+  /* concat in chunk order (tracer.rs:306-307) */
+  or_segs* out = (or_segs*)calloc(1, sizeof(or_segs));
+  uint64_t total = 0;
+  for (size_t i = 0; i < n_chunks; i++) total += J.chunk_out[i].n;
+  if (!J.failed && out && segs_reserve(out, total ? total : 1)) {
+    for (size_t i = 0; i < n_chunks; i++) {
+      or_segs* s = &J.chunk_out[i];
+      memcpy(out->pos + 4 * out->n, s->pos, s->n * 4 * sizeof(double));
+      memcpy(out->color + 4 * out->n, s->color, s->n * 4 * sizeof(float));
+      memcpy(out->key + out->n, s->key, s->n * sizeof(uint64_t));
+      out->n += s->n;
+    }
+  } else {
+    or_segs_free(out);
+    out = NULL;
+  }
+  for (size_t i = 0; i < n_chunks; i++) {
+    free(J.chunk_out[i].pos);
+    free(J.chunk_out[i].color);
+    free(J.chunk_out[i].key);
+  }
+  free(J.chunk_out);
+  free(chunks);
+  free(light_ior);
+  free(light_base);
+  return out;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* accumulation                                                                          */
+/* ------------------------------------------------------------------------------------ */
+/*
+ * DEFINED rasterisation rule (the reference uses the driver's aliased 1-px LineList raster
+ * with dst.rgb += src.rgb and dst.a += src.a * src.a, mod.rs:57-73):
+ *   world -> pixel: X = (x - left) * W / (right - left), Y = (top - y) * H / (top - bottom)
+ *   major axis = x if |dX| >= |dY| else y; zero-length segments deposit nothing.
+ *   x-major: one sample per pixel column whose centre i + 0.5 lies in the half-open span
+ *   from X0 (included) to X1 (excluded); s = (i + 0.5 - X0) / dX; Y = Y0 + s dY;
+ *   colour = c0 + s (c1 - c0); v = Y - 0.5, j = floor(v), f = v - j; pixel (i, j) gets
+ *   weight 1 - f and pixel (i, j + 1) weight f. y-major is the same with axes swapped.
+ *   A deposit of weight w adds w * rgb to RGB and w * a * a to alpha.
+ */
+typedef struct acc_job {
+  const void* pos;
+  int pos_is_f32;
+  const float* c0;
+  const float* c1;
+  uint64_t n;
+  const lg_view* view;
+  void* accum;
+  int accum_is_f32;
+  int band_lo, band_hi; /* rows [lo, hi) owned by this thread */
+} acc_job;
+
+static inline void acc_deposit(acc_job* J, int px, int py, double w, const double col[4]) {
+  uint32_t W = J->view->width;
+  if (px < 0 || px >= (int)W || py < J->band_lo || py >= J->band_hi) return;
+  size_t idx = ((size_t)py * W + (size_t)px) * 4;
+  if (J->accum_is_f32) {
+    float* a = (float*)J->accum + idx;
+    float wf = (float)w;
+    a[0] += wf * (float)col[0];
+    a[1] += wf * (float)col[1];
+    a[2] += wf * (float)col[2];
+    a[3] += wf * (float)col[3] * (float)col[3];
+  } else {
+    double* a = (double*)J->accum + idx;
+    a[0] += w * col[0];
+    a[1] += w * col[1];
+    a[2] += w * col[2];
+    a[3] += w * col[3] * col[3];
+  }
+}
+
+static void seg_pixels(const lg_view* v, double x0, double y0, double x1, double y1,
+                       double* X0, double* Y0, double* X1, double* Y1) {
+  double sx = (double)v->width / (v->right - v->left);
+  double sy = (double)v->height / (v->top - v->bottom);
+  *X0 = (x0 - v->left) * sx;
+  *Y0 = (v->top - y0) * sy;
+  *X1 = (x1 - v->left) * sx;
+  *Y1 = (v->top - y1) * sy;
+}
+
+/* sample index range [k_lo, k_hi] (inclusive) of major-axis coordinate M0 -> M1, cut to
+ * the pixels [0, limit) */
+static int major_range(double M0, double M1, long limit, long* k_lo, long* k_hi) {
+  double lo, hi;
+  if (M1 > M0) {
+    lo = ceil(M0 - 0.5);
+    hi = ceil(M1 - 0.5) - 1.0;
+  } else if (M1 < M0) {
+    lo = floor(M1 - 0.5) + 1.0;
+    hi = floor(M0 - 0.5);
+  } else {
+    return 0;
+  }
+  if (!(lo <= hi)) return 0;
+  if (lo < 0.0) lo = 0.0;
+  if (hi > (double)(limit - 1)) hi = (double)(limit - 1);
+  if (lo > hi) return 0;
+  *k_lo = (long)lo;
+  *k_hi = (long)hi;
+  return 1;
+}
+
+static void acc_segment(acc_job* J, uint64_t s) {
+  double x0, y0, x1, y1;
+  if (J->pos_is_f32) {
+    const float* p = (const float*)J->pos + 4 * s;
+    x0 = p[0]; y0 = p[1]; x1 = p[2]; y1 = p[3];
+  } else {
+    const double* p = (const double*)J->pos + 4 * s;
+    x0 = p[0]; y0 = p[1]; x1 = p[2]; y1 = p[3];
+  }
+  double X0, Y0, X1, Y1;
+  seg_pixels(J->view, x0, y0, x1, y1, &X0, &Y0, &X1, &Y1);
+  if (!isfinite(X0) || !isfinite(Y0) || !isfinite(X1) || !isfinite(Y1)) return;
+  double dX = X1 - X0, dY = Y1 - Y0;
+  const float* ca = J->c0 + 4 * s;
+  const float* cb = J->c1 ? J->c1 + 4 * s : ca;
+  int xmajor = fabs(dX) >= fabs(dY);
+  double M0 = xmajor ? X0 : Y0, M1 = xmajor ? X1 : Y1;
+  double m0 = xmajor ? Y0 : X0, dM = xmajor ? dX : dY, dm = xmajor ? dY : dX;
+  long klo, khi;
+  long limit = xmajor ? (long)J->view->width : (long)J->view->height;
+  if (!major_range(M0, M1, limit, &klo, &khi)) return;
+  if (!xmajor) {
+    /* rows are the major axis: only this thread's band */
+    if (klo < J->band_lo) klo = J->band_lo;
+    if (khi > J->band_hi - 1) khi = J->band_hi - 1;
+  } else {
+    /* rows are the minor axis: the samples whose deposit rows can meet the band have
+     * m in [band_lo - 0.5, band_hi + 0.5); widen by one column on each side and let the
+     * per-deposit test below decide exactly */
+    double lo_m = (double)J->band_lo - 0.5, hi_m = (double)J->band_hi + 0.5;
+    if (dm == 0.0) {
+      if (m0 < lo_m || m0 >= hi_m) return;
+    } else {
+      double ka = M0 - 0.5 + (lo_m - m0) * dM / dm;
+      double kb = M0 - 0.5 + (hi_m - m0) * dM / dm;
+      double kmin = ka < kb ? ka : kb, kmax = ka < kb ? kb : ka;
+      if (kmin < (double)klo - 2.0) kmin = (double)klo - 2.0;
+      if (kmax > (double)khi + 2.0) kmax = (double)khi + 2.0;
+      if (kmin > kmax) return;
+      long a = (long)floor(kmin) - 1, b = (long)ceil(kmax) + 1;
+      if (a > klo) klo = a;
+      if (b < khi) khi = b;
+    }
+  }
+  for (long k = klo; k <= khi; k++) {
+    double s = ((double)k + 0.5 - M0) / dM;
+    double m = m0 + s * dm;
+    double v = m - 0.5;
+    double jf = floor(v);
+    double f = v - jf;
+    long j = (long)jf;
+    double col[4];
+    for (int c = 0; c < 4; c++) col[c] = (double)ca[c] + s * ((double)cb[c] - (double)ca[c]);
+    if (xmajor) {
+      if (j + 1 < J->band_lo || j >= J->band_hi) continue;
+      acc_deposit(J, (int)k, (int)j, 1.0 - f, col);
+      acc_deposit(J, (int)k, (int)(j + 1), f, col);
+    } else {
+      acc_deposit(J, (int)j, (int)k, 1.0 - f, col);
+      acc_deposit(J, (int)(j + 1), (int)k, f, col);
+    }
+  }
+}
+
+static void* acc_worker(void* arg) {
+  acc_job* J = (acc_job*)arg;
+  for (uint64_t s = 0; s < J->n; s++) acc_segment(J, s);
+  return NULL;
+}
+
+static void accumulate_generic(const void* pos, int pos_f32, const float* c0, const float* c1,
+                               uint64_t n, const lg_view* view, void* accum, int acc_f32,
+                               int n_threads) {
+  if (n_threads < 1) n_threads = 1;
+  int H = (int)view->height;
+  if (n_threads > H) n_threads = H;
+  acc_job* jobs = (acc_job*)calloc((size_t)n_threads, sizeof(acc_job));
+  pthread_t* th = (pthread_t*)calloc((size_t)n_threads, sizeof(pthread_t));
+  for (int t = 0; t < n_threads; t++) {
+    acc_job* J = &jobs[t];
+    J->pos = pos;
+    J->pos_is_f32 = pos_f32;
+    J->c0 = c0;
+    J->c1 = c1;
+    J->n = n;
+    J->view = view;
+    J->accum = accum;
+    J->accum_is_f32 = acc_f32;
+    J->band_lo = (int)((long)H * t / n_threads);
+    J->band_hi = (int)((long)H * (t + 1) / n_threads);
+  }
+  if (n_threads == 1) {
+    acc_worker(&jobs[0]);
+  } else {
+    for (int t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, acc_worker, &jobs[t]);
+    for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
+  }
+  free(th);
+  free(jobs);
+}
+
+void or_accumulate_f64(const double* pos, const float* color0, const float* color1, uint64_t n,
+                       const lg_view* view, double* accum, int n_threads) {
+  accumulate_generic(pos, 0, color0, color1, n, view, accum, 0, n_threads);
+}
+
+void or_accumulate_f32(const float* pos, const float* color0, const float* color1, uint64_t n,
+                       const lg_view* view, float* accum, int n_threads) {
+  accumulate_generic(pos, 1, color0, color1, n, view, accum, 1, n_threads);
+}
+
+uint64_t or_sample_count(const double* pos, uint64_t n, const lg_view* view) {
+  uint64_t total = 0;
+  for (uint64_t s = 0; s < n; s++) {
+    const double* p = pos + 4 * s;
+    double X0, Y0, X1, Y1;
+    seg_pixels(view, p[0], p[1], p[2], p[3], &X0, &Y0, &X1, &Y1);
+    if (!isfinite(X0) || !isfinite(Y0) || !isfinite(X1) || !isfinite(Y1)) continue;
+    int xmajor = fabs(X1 - X0) >= fabs(Y1 - Y0);
+    long klo, khi;
+    long limit = xmajor ? (long)view->width : (long)view->height;
+    if (!major_range(xmajor ? X0 : Y0, xmajor ? X1 : Y1, limit, &klo, &khi)) continue;
+    total += (uint64_t)(khi - klo + 1);
+  }
+  return total;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* FP16 store (round to nearest even) and resolve                                        */
+/* ------------------------------------------------------------------------------------ */
+uint16_t or_half_from_double(double x) {
+  /* IEEE binary16 from a double, round to nearest even. */
+  uint16_t sign = 0;
+  if (signbit(x)) {
+    sign = 0x8000;
+    x = -x;
+  }
+  if (isnan(x)) return sign | 0x7e00;
+  if (x >= 65520.0) return sign | 0x7c00; /* rounds to infinity */
+  if (x < 0x1p-25) {
+    /* below half the smallest subnormal: zero, except exactly... (x < 2^-25 rounds to 0) */
+    return sign;
+  }
+  int e;
+  double m = frexp(x, &e); /* x = m * 2^e, m in [0.5, 1) */
+  int exp_h = e - 1;        /* x = (2m) * 2^(e-1), 2m in [1, 2) */
+  if (exp_h < -14) {
+    /* subnormal: value = k * 2^-24 */
+    double k = x * 0x1p24;
+    double r = nearbyint(k); /* default rounding mode: nearest even */
+    return sign | (uint16_t)r; /* r == 1024 yields the smallest normal encoding */
+  }
+  double frac = (2.0 * m - 1.0) * 1024.0; /* 10 fraction bits */
+  double r = nearbyint(frac);
+  int ex = exp_h + 15;
+  if (r >= 1024.0) {
+    r = 0.0;
+    ex += 1;
+  }
+  if (ex >= 31) return sign | 0x7c00;
+  return sign | (uint16_t)(ex << 10) | (uint16_t)r;
+}
+
+double or_double_from_half(uint16_t h) {
+  int sign = (h >> 15) & 1;
+  int ex = (h >> 10) & 0x1f;
+  int fr = h & 0x3ff;
+  double v;
+  if (ex == 0)
+    v = ldexp((double)fr, -24);
+  else if (ex == 31)
+    v = fr ? NAN : INFINITY;
+  else
+    v = ldexp(1.0 + fr / 1024.0, ex - 15);
+  return sign ? -v : v;
+}
+
+void or_resolve_f64(const double* accum, uint64_t n_pixels, uint16_t* out) {
+  for (uint64_t i = 0; i < n_pixels; i++) {
+    out[4 * i + 0] = or_half_from_double(accum[4 * i + 0]);
+    out[4 * i + 1] = or_half_from_double(accum[4 * i + 1]);
+    out[4 * i + 2] = or_half_from_double(accum[4 * i + 2]);
+    out[4 * i + 3] = or_half_from_double(accum[4 * i + 3] + 1.0);
+  }
+}
+
+void or_resolve_f32(const float* accum, uint64_t n_pixels, uint16_t* out) {
+  for (uint64_t i = 0; i < n_pixels; i++) {
+    out[4 * i + 0] = or_half_from_double((double)accum[4 * i + 0]);
+    out[4 * i + 1] = or_half_from_double((double)accum[4 * i + 1]);
+    out[4 * i + 2] = or_half_from_double((double)accum[4 * i + 2]);
+    out[4 * i + 3] = or_half_from_double((double)(accum[4 * i + 3] + 1.0f));
+  }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* string mod (string_mod.rs:33-158)                                                     */
+/* ------------------------------------------------------------------------------------ */
+static uint64_t wrapping_pow(uint64_t b, uint64_t e) {
+  /* u64::pow in a release build wraps modulo 2^64 */
+  uint64_t r = 1;
+  while (e) {
+    if (e & 1) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return r;
+}
+
+static void sm_color(uint64_t ix, const float base[4], const or_mod_rem_color* mrc, int n_mrc,
+                     float out[4]) {
+  /* string_mod.rs:124-150 */
+  float acc[4] = {0, 0, 0, 0};
+  int k = 0;
+  for (int i = 0; i < n_mrc; i++) {
+    if (mrc[i].modulo && (ix % mrc[i].modulo) == mrc[i].rem) {
+      for (int c = 0; c < 4; c++) acc[c] += mrc[i].color[c];
+      k++;
+    }
+  }
+  if (k == 0) {
+    memcpy(out, base, 4 * sizeof(float));
+  } else {
+    for (int c = 0; c < 4; c++) out[c] = acc[c] / (float)k;
+  }
+}
+
+void or_string_mod(uint64_t modulo, uint64_t num, uint64_t turns, int mode, double scale,
+                   const float color[4], const or_mod_rem_color* mrc, int n_mrc, double* pos,
+                   float* color0, float* color1) {
+  for (uint64_t iix = 0; iix < modulo; iix++) {
+    uint64_t ix;
+    switch (mode) {
+      case OR_SM_ADD: ix = (iix + num) % modulo; break;
+      case OR_SM_MUL: ix = (iix * num) % modulo; break;
+      case OR_SM_POW: ix = wrapping_pow(iix, num) % modulo; break;
+      default: ix = wrapping_pow(num, iix) % modulo; break;
+    }
+    /* Curve::Circle point n: angle = (turns * n) * TAU / m (string_mod.rs:45-54) */
+    double a0 = (double)(turns * iix) * (2.0 * M_PI) / (double)modulo;
+    double a1 = (double)(turns * ix) * (2.0 * M_PI) / (double)modulo;
+    double* p = pos + 4 * iix;
+    p[0] = scale * cos(a0);
+    p[1] = scale * sin(a0);
+    p[2] = scale * cos(a1);
+    p[3] = scale * sin(a1);
+    sm_color(iix, color, mrc, n_mrc, color0 + 4 * iix);
+    sm_color(ix, color, mrc, n_mrc, color1 + 4 * iix);
+  }
+}
+
+double or_now_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}

@@ -1,0 +1,246 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/build/liblgoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py, as the checker. The product (light_garden_amd) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liblgoracle.so")
+
+_lib = None
+
+
+def build():
+    """Compiles the oracle with its Makefile (gcc)."""
+    r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB_PATH
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_double), ("p", C.c_double * 2), ("n", C.c_double * 2)]
+
+
+class ModRemColor(C.Structure):
+    _fields_ = [("modulo", C.c_uint64), ("rem", C.c_uint64), ("color", C.c_float * 4)]
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    D2 = C.POINTER(C.c_double)
+    sig = {
+        "or_intersect_object": (C.c_int, [P, D2, D2, C.POINTER(Hit), C.c_int]),
+        "or_contains_object": (C.c_int, [P, D2]),
+        "or_object_is_closed": (C.c_int, [P]),
+        "or_refract": (C.c_int, [D2, D2, C.c_double, C.c_double, D2, D2, D2]),
+        "or_reflect": (None, [D2, D2, D2]),
+        "or_canvas_exit": (C.c_int, [D2, D2, D2, D2]),
+        "or_light_ray": (None, [P, C.c_uint64, D2, D2]),
+        "or_initial_ior": (C.c_double, [P, C.c_uint32, D2]),
+        "or_trace": (P, [P, C.c_uint32, P, C.c_uint32, P, C.c_uint64, C.c_int, C.c_int]),
+        "or_segs_count": (C.c_uint64, [P]),
+        "or_segs_copy": (None, [P, P, P, P, P]),
+        "or_segs_free": (None, [P]),
+        "or_accumulate_f64": (None, [P, P, P, C.c_uint64, P, P, C.c_int]),
+        "or_accumulate_f32": (None, [P, P, P, C.c_uint64, P, P, C.c_int]),
+        "or_sample_count": (C.c_uint64, [P, C.c_uint64, P]),
+        "or_resolve_f64": (None, [P, C.c_uint64, P]),
+        "or_resolve_f32": (None, [P, C.c_uint64, P]),
+        "or_half_from_double": (C.c_uint16, [C.c_double]),
+        "or_double_from_half": (C.c_double, [C.c_uint16]),
+        "or_string_mod": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_double, P,
+                                 C.POINTER(ModRemColor), C.c_int, P, P, P]),
+        "or_now_seconds": (C.c_double, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _d2(v):
+    return (C.c_double * 2)(float(v[0]), float(v[1]))
+
+
+# ---- geometry -----------------------------------------------------------------------------
+def intersect(obj, o, d):
+    """All hits of ray (o, d) with an lg_object (LgObject): list of (t, p, n)."""
+    hits = (Hit * 48)()
+    k = load().or_intersect_object(C.byref(obj), _d2(o), _d2(d), hits, 48)
+    return [(hits[i].t, (hits[i].p[0], hits[i].p[1]), (hits[i].n[0], hits[i].n[1])) for i in range(k)]
+
+
+def contains(obj, p):
+    return bool(load().or_contains_object(C.byref(obj), _d2(p)))
+
+
+def refract(d, n, n1, n2):
+    """-> (reflected dir, refracted dir or None, reflectance)"""
+    rl, rr, R = _d2((0, 0)), _d2((0, 0)), C.c_double()
+    ok = load().or_refract(_d2(d), _d2(n), n1, n2, rl, rr, C.byref(R))
+    return (rl[0], rl[1]), ((rr[0], rr[1]) if ok else None), R.value
+
+
+def reflect(d, n):
+    out = _d2((0, 0))
+    load().or_reflect(_d2(d), _d2(n), out)
+    return (out[0], out[1])
+
+
+def canvas_exit(canvas, o, d):
+    cv = (C.c_double * 4)(*[float(c) for c in canvas])
+    p = _d2((0, 0))
+    ok = load().or_canvas_exit(cv, _d2(o), _d2(d), p)
+    return (p[0], p[1]) if ok else None
+
+
+def light_ray(light, i):
+    o, d = _d2((0, 0)), _d2((0, 0))
+    load().or_light_ray(C.byref(light), int(i), o, d)
+    return (o[0], o[1]), (d[0], d[1])
+
+
+def initial_ior(objs, n_obj, p):
+    return load().or_initial_ior(objs, n_obj, _d2(p))
+
+
+# ---- trace --------------------------------------------------------------------------------
+def trace(objs, n_obj, lights, n_lights, params, ray_stride=1, threads=1, chunk=100):
+    """Returns (pos (S,4) float64, color (S,4) float32, key (S,) uint64) in the reference's
+    order."""
+    lib = load()
+    h = lib.or_trace(objs, n_obj, lights, n_lights, C.byref(params), int(ray_stride), int(threads),
+                     int(chunk))
+    if not h:
+        raise MemoryError("oracle trace failed")
+    try:
+        n = lib.or_segs_count(h)
+        pos = np.empty((n, 4), np.float64)
+        col = np.empty((n, 4), np.float32)
+        key = np.empty(n, np.uint64)
+        lib.or_segs_copy(h, pos.ctypes.data, col.ctypes.data, None, key.ctypes.data)
+    finally:
+        lib.or_segs_free(h)
+    return pos, col, key
+
+
+def trace_scene(scene, ray_stride=1, threads=1):
+    """Traces a light_garden_amd.configs.Scene."""
+    from light_garden_amd import lights_array, objects_array
+    objs = objects_array(scene.objects)
+    lights = lights_array(scene.lights)
+    return trace(objs, len(scene.objects), lights, len(scene.lights), scene.params(), ray_stride, threads)
+
+
+# ---- accumulation -------------------------------------------------------------------------
+def accumulate_f64(pos, color0, color1, view, threads=1):
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 4)
+    c0 = np.ascontiguousarray(color0, np.float32).reshape(-1, 4)
+    c1 = None if color1 is None else np.ascontiguousarray(color1, np.float32).reshape(-1, 4)
+    acc = np.zeros((view.height, view.width, 4), np.float64)
+    load().or_accumulate_f64(pos.ctypes.data, c0.ctypes.data, None if c1 is None else c1.ctypes.data,
+                             pos.shape[0], C.byref(view), acc.ctypes.data, int(threads))
+    return acc
+
+
+def accumulate_f32(pos, color0, color1, view, threads=1, acc=None):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+    c0 = np.ascontiguousarray(color0, np.float32).reshape(-1, 4)
+    c1 = None if color1 is None else np.ascontiguousarray(color1, np.float32).reshape(-1, 4)
+    if acc is None:
+        acc = np.zeros((view.height, view.width, 4), np.float32)
+    load().or_accumulate_f32(pos.ctypes.data, c0.ctypes.data, None if c1 is None else c1.ctypes.data,
+                             pos.shape[0], C.byref(view), acc.ctypes.data, int(threads))
+    return acc
+
+
+def sample_count(pos, view):
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 4)
+    return load().or_sample_count(pos.ctypes.data, pos.shape[0], C.byref(view))
+
+
+def resolve_f64(acc):
+    acc = np.ascontiguousarray(acc, np.float64)
+    out = np.empty(acc.shape, np.uint16)
+    load().or_resolve_f64(acc.ctypes.data, acc.size // 4, out.ctypes.data)
+    return out
+
+
+def resolve_f32(acc):
+    acc = np.ascontiguousarray(acc, np.float32)
+    out = np.empty(acc.shape, np.uint16)
+    load().or_resolve_f32(acc.ctypes.data, acc.size // 4, out.ctypes.data)
+    return out
+
+
+def half_bits(x):
+    return load().or_half_from_double(float(x))
+
+
+def half_value(h):
+    return load().or_double_from_half(int(h))
+
+
+def string_mod(modulo, num, turns=1, mode=1, scale=1.0, color=(1, 1, 1, 1), mrc=()):
+    pos = np.empty((modulo, 4), np.float64)
+    c0 = np.empty((modulo, 4), np.float32)
+    c1 = np.empty((modulo, 4), np.float32)
+    arr = (ModRemColor * max(1, len(mrc)))()
+    for i, (m, r, c) in enumerate(mrc):
+        arr[i].modulo, arr[i].rem = m, r
+        arr[i].color[:] = [float(x) for x in c]
+    col = (C.c_float * 4)(*[float(x) for x in color])
+    load().or_string_mod(modulo, num, turns, mode, scale, col, arr, len(mrc), pos.ctypes.data,
+                         c0.ctypes.data, c1.ctypes.data)
+    return pos, c0, c1
+
+
+def now():
+    return load().or_now_seconds()
+
+
+# ---- comparison helpers -------------------------------------------------------------------
+def rel_l1(img_rgb, ref_rgb):
+    """sum |img - ref| / sum |ref| over the RGB channels."""
+    img = np.asarray(img_rgb, np.float64)[..., :3]
+    ref = np.asarray(ref_rgb, np.float64)[..., :3]
+    den = np.abs(ref).sum()
+    return float(np.abs(img - ref).sum() / den) if den > 0 else float(np.abs(img).sum())
+
+
+def decode_half(bits):
+    return np.asarray(bits, np.uint16).view(np.float16).astype(np.float64)
+
+
+def match_segments(key_a, pos_a, key_b, pos_b, extent=1.0, diverge_tol=1e-3):
+    """Matches two segment sets by key. Returns a dict with the matched count, keys only in
+    one set, the endpoint error of matched segments (|dp|_inf / extent) and the count of
+    matched segments whose end points differ by more than diverge_tol (branch flips)."""
+    ka = np.asarray(key_a, np.uint64)
+    kb = np.asarray(key_b, np.uint64)
+    common, ia, ib = np.intersect1d(ka, kb, assume_unique=False, return_indices=True)
+    err = np.abs(np.asarray(pos_a, np.float64)[ia] - np.asarray(pos_b, np.float64)[ib]).max(axis=1) / extent \
+        if common.size else np.zeros(0)
+    diverged = err > diverge_tol
+    good = err[~diverged]
+    return {
+        "n_a": int(ka.size), "n_b": int(kb.size), "matched": int(common.size),
+        "only_a": int(ka.size - common.size), "only_b": int(kb.size - common.size),
+        "diverged": int(diverged.sum()),
+        "max_err": float(good.max()) if good.size else 0.0,
+        "p99_err": float(np.percentile(good, 99)) if good.size else 0.0,
+    }

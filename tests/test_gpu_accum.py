@@ -1,0 +1,139 @@
+"""GPU accumulation + resolve parity against the FP64 oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+import light_garden_amd as lg
+from light_garden_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_accum(ctx, pos, c0, c1, width, height, bounds=(-1.0, 1.0, -1.0, 1.0)):
+    segs = lg.Segments(ctx, capacity=max(1, len(pos)), vertex_colors=c1 is not None)
+    segs.upload(pos, c0, c1)
+    img = lg.Image(ctx, width, height, bounds)
+    img.accumulate(segs)
+    img.resolve()
+    return img
+
+
+def test_accumulate_traced_segments(ctx, orc):
+    """The plain small scene's oracle segments: FP32 accumulator vs FP64 accumulation, and
+    the RGBA16F target within 1e-3 relative L1."""
+    scene = configs.plain_small()
+    pos, col, _ = orc.trace_scene(scene, threads=8)
+    img = gpu_accum(ctx, pos, col, None, scene.width, scene.height)
+    ref = orc.accumulate_f64(pos, col, None, scene.view(), threads=8)
+    acc = img.accum()
+    e_acc = orc.rel_l1(acc, ref)
+    e_img = orc.rel_l1(img.rgba(), ref)
+    print("accum rel L1 %.3e  image rel L1 %.3e" % (e_acc, e_img))
+    assert e_acc < 2e-4
+    assert e_img < 1e-3
+    # alpha: sum of w * a^2 plus the clear value 1
+    np.testing.assert_allclose(acc[..., 3].sum(), ref[..., 3].sum(), rtol=1e-4)
+
+
+def test_resolve_bits_match_oracle(ctx, orc):
+    """Resolve is FP32 -> FP16 round-to-nearest-even, alpha + 1: bit-exact vs the oracle's
+    resolve of the same FP32 accumulator."""
+    scene = configs.plain_small(num_rays=4000)
+    pos, col, _ = orc.trace_scene(scene)
+    img = gpu_accum(ctx, pos, col, None, 512, 512)
+    ref_bits = orc.resolve_f32(img.accum())
+    assert np.array_equal(img.rgba16f(), ref_bits)
+
+
+@pytest.mark.parametrize("k_list,n,side", [((2, 3, 5), 10000, 1024), ((7, 11), 5000, 512)])
+def test_string_mod_accumulation(ctx, orc, k_list, n, side):
+    """String-mod segments (per-vertex colours path) at reduced size."""
+    spec = configs.StringModSpec(modulo=n, mults=k_list, width=side, height=side)
+    pos, col = configs.string_mod_segments(spec)
+    img = gpu_accum(ctx, pos, col, col, side, side)
+    view = abi.LgView(-1.0, 1.0, -1.0, 1.0, side, side)
+    ref = orc.accumulate_f64(pos, col, col, view, threads=8)
+    e = orc.rel_l1(img.rgba(), ref)
+    print("string mod rel L1 %.3e" % e)
+    assert e < 1e-3
+
+
+def test_vertex_colour_interpolation(ctx, orc):
+    rng = np.random.default_rng(5)
+    n = 2000
+    pos = rng.uniform(-1.2, 1.2, (n, 4))
+    c0 = rng.uniform(0, 0.01, (n, 4)).astype(np.float32)
+    c1 = rng.uniform(0, 0.01, (n, 4)).astype(np.float32)
+    img = gpu_accum(ctx, pos, c0, c1, 300, 200, (-1.0, 1.0, -0.7, 0.6))
+    view = abi.LgView(-1.0, 1.0, -0.7, 0.6, 300, 200)
+    ref = orc.accumulate_f64(pos.astype(np.float32).astype(np.float64), c0, c1, view)
+    assert orc.rel_l1(img.accum(), ref) < 1e-5
+
+
+def test_edge_cases(ctx, orc):
+    """Zero-length, axis-aligned, pixel-centre-aligned, off-screen, partly off-screen, exact
+    diagonals, NaN end points and segments spanning many tiles."""
+    W = H = 256
+    px = 2.0 / W
+    segs = [
+        (0.1, 0.1, 0.1, 0.1),                      # zero length: nothing
+        (-0.5, 0.2, 0.5, 0.2),                     # horizontal
+        (0.3, -0.9, 0.3, 0.9),                     # vertical
+        (-1.0 + 10.5 * px, 0.0, -1.0 + 20.5 * px, 0.0),  # ends on pixel centres
+        (-0.8, -0.8, 0.8, 0.8),                    # exact diagonal
+        (-3.0, 0.5, 3.0, 0.4),                     # crosses the whole image
+        (2.0, 2.0, 3.0, 3.0),                      # off-screen
+        (0.9, 0.9, 1.5, 1.3),                      # partly off-screen
+        (-0.99, -0.99, 0.99, 0.98),                # many tiles
+        (0.0, 0.0, 1e-7, 0.0),                     # sub-pixel
+        (float("nan"), 0.0, 0.5, 0.5),             # NaN: nothing
+    ]
+    pos = np.array(segs, np.float64)
+    col = np.tile(np.array([[0.25, 0.5, 0.75, 0.5]], np.float32), (len(segs), 1))
+    img = gpu_accum(ctx, pos, col, None, W, H)
+    view = abi.LgView(-1.0, 1.0, -1.0, 1.0, W, H)
+    ref = orc.accumulate_f64(pos, col, None, view)
+    acc = img.accum()
+    np.testing.assert_allclose(acc, ref, atol=1e-5, rtol=1e-5)
+    # a horizontal segment across whole columns deposits exactly 1 per column
+    one = np.array([(-1.0 + 2 * px, 0.5 * px, -1.0 + 6 * px, 0.5 * px)], np.float64)
+    img1 = gpu_accum(ctx, one, col[:1], None, W, H)
+    a1 = img1.accum()
+    assert abs(a1[..., 0].sum() / 0.25 - 4.0) < 1e-5
+
+
+def test_accumulate_adds_and_empty(ctx):
+    pos = np.array([[-0.5, 0.0, 0.5, 0.0]], np.float64)
+    col = np.array([[0.1, 0.2, 0.3, 0.4]], np.float32)
+    segs = lg.Segments(ctx, 4)
+    segs.upload(pos, col)
+    img = lg.Image(ctx, 64, 64)
+    img.accumulate(segs)
+    a1 = img.accum()
+    img.accumulate(segs)
+    a2 = img.accum()
+    np.testing.assert_allclose(a2, 2 * a1, rtol=1e-6)
+    empty = lg.Segments(ctx, 4)
+    empty.upload(np.zeros((0, 4)), np.zeros((0, 4)))
+    img.accumulate(empty)
+    np.testing.assert_allclose(img.accum(), a2)
+    img.clear()
+    assert not img.accum().any()
+
+
+def test_render_plain_small(ctx, orc):
+    """One frame (trace + accumulate + resolve) of the plain small scene vs the FP64 oracle's
+    segments accumulated in FP64."""
+    scene = configs.plain_small()
+    tr = lg.Tracer(ctx, scene.canvas)
+    for o in scene.objects:
+        tr.push_object(o)
+    for l in scene.lights:
+        tr.push_light(l)
+    segs = lg.Segments(ctx)
+    img = lg.Image(ctx, scene.width, scene.height, scene.canvas)
+    tr.render(segs, img)
+    rpos, rcol, _ = orc.trace_scene(scene, threads=8)
+    ref = orc.accumulate_f64(rpos, rcol, None, scene.view(), threads=8)
+    e = orc.rel_l1(img.rgba(), ref)
+    print("plain small image rel L1 %.3e" % e)
+    assert e < 1e-3
