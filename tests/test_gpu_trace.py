@@ -123,3 +123,57 @@ def test_trace_capacity_regrow(ctx, orc):
     tr.trace(segs)
     _, _, rkey = orc.trace_scene(scene)
     assert abs(segs.count() - len(rkey)) <= 0.01 * len(rkey)
+
+
+def shape_zoo():
+    """Every shape kind and boolean op the GPU path handles, overlapping each other."""
+    from light_garden_amd import geo as G
+    rng = np.random.default_rng(31)
+    objs = []
+    prims = lambda: [G.Circle((rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1)), rng.uniform(0.1, 0.25)),
+                     G.Rect((rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1)), G.rotation(rng.uniform(0, 3)),
+                            rng.uniform(0.1, 0.4), rng.uniform(0.1, 0.4)),
+                     G.Ellipse((rng.uniform(-0.1, 0.1), 0.0), rng.uniform(0.1, 0.3), rng.uniform(0.05, 0.2),
+                               G.rotation(rng.uniform(0, 3))),
+                     G.ConvexPolygon.new_convex_hull([(rng.uniform(-0.25, 0.25), rng.uniform(-0.25, 0.25))
+                                                      for _ in range(6)])]
+    centres = [(x, y) for x in (-0.6, -0.2, 0.2, 0.6) for y in (-0.6, -0.2, 0.2, 0.6)]
+    k = 0
+    for op in (abi.LOGIC_AND, abi.LOGIC_OR, abi.LOGIC_AND_NOT):
+        for i in range(4):
+            a, b = prims()[i], prims()[(i + 1 + op) % 4]
+            c = centres[k % len(centres)]
+            k += 1
+            objs.append(lg.Object(G.Logic(op, a, b, c, G.rotation(rng.uniform(0, 3))),
+                                  lg.Material(rng.uniform(1.3, 1.8))))
+    for p in prims():
+        c = centres[k % len(centres)]
+        k += 1
+        objs.append(lg.Object(p.translated(*c), lg.Material(rng.uniform(1.3, 1.8))))
+    objs.append(lg.Object.new_mirror((-0.9, 0.85), (0.9, 0.8)))
+    objs.append(lg.Object.new_curved_mirror(configs.arc_bezier((0.0, 0.0), 0.95, -0.6, 0.6)))
+    mirrored_circle = lg.Object.new_circle((0.0, -0.85), 0.08)
+    mirrored_circle.material_opt = None  # a circle without material reflects
+    objs.append(mirrored_circle)
+    lights = [lg.PointLight((0.01, 0.02), 4000, [0.02, 0.02, 0.02, 0.02]),
+              lg.SpotLight((-0.95, -0.95), 0.6, (1.0, 1.0), 3000, [0.03, 0.01, 0.02, 0.02]),
+              lg.DirectionalLight([0.01, 0.03, 0.02, 0.02], 3000, lg.LineSegment((-0.9, 0.95), (0.9, 0.95)))]
+    # the left normal of b - a = (1.8, 0) points up, out of the canvas: reverse the band
+    lights[2].start = lg.LineSegment((0.9, 0.95), (-0.9, 0.95))
+    return configs.Scene(objs, lights, max_bounce=8, width=256, height=256, name="shape_zoo")
+
+
+@pytest.mark.parametrize("fp64", [True, False], ids=["fp64", "fp32"])
+def test_trace_shape_zoo(ctx, orc, fp64):
+    """And/Or/AndNot over circle/rect/ellipse/polygon operands, every primitive kind, a
+    mirror without material: the kernel's interval boolean rule and chords agree with the
+    oracle's hit-filter rule."""
+    scene = shape_zoo()
+    pos, col, key = gpu_trace(ctx, scene, fp64=fp64)
+    rpos, rcol, rkey = orc.trace_scene(scene, threads=8)
+    m = orc.match_segments(key, pos, rkey, rpos)
+    print(m)
+    assert m["n_b"] > 20000
+    assert m["matched"] >= 0.999 * m["n_b"]
+    assert m["diverged"] <= 0.001 * m["n_b"]
+    assert m["max_err"] < (1e-9 if fp64 else 1e-6)
