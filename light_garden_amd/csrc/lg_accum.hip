@@ -275,9 +275,14 @@ __device__ __forceinline__ double bcast_d(double v, int lane) {
 __device__ __forceinline__ int fx(float v, float scale) { return (int)__builtin_rintf(v * scale); }
 
 // One persistent workgroup per (tile, pair range) item. Each wave takes 64 pairs at a time:
-// lane l loads pair l's segment and computes its sample range inside the tile; the wave then
-// walks those pairs one after another with one sample per lane, reading the pair's setup
-// from scalar registers.
+// lane l loads pair l's segment and computes its sample range inside the tile plus the
+// tile-relative FP32 parameters of its samples; the wave then walks those pairs one after
+// another with one sample per lane, reading the pair's setup from scalar registers.
+//
+// Sample positions are anchored on the tile column (the first major-axis pixel of the
+// tile), which is the same for every tile that shares a sample: m(q) = m_anchor + q * slope,
+// q = k - maj0, in FP32 relative to the tile's minor origin. Two tiles sharing a sample
+// differ only by the FP32 rounding of their anchors (< 1e-5 px).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(
     const float4* __restrict__ pos, const float4* __restrict__ col0,
@@ -304,88 +309,100 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (uint32_t base = W.begin + wave * 64; base < W.end; base += kWaves * 64) {
       // ---- per-lane setup of one pair ----
-      uint32_t pi = base + lane;
-      int n_samp = 0;
-      int ka = 0, xm = 0, vc = 0;
-      double M0 = 0.0, m0 = 0.0, slope = 0.0, inv_dM = 0.0;
+      const uint32_t pi = base + lane;
+      int packed = 0;  // n | q0 << 8 | xmajor << 16 | vertex colours << 17
+      float m_anchor = 0.f, slope = 0.f, s_anchor = 0.f, ds = 0.f;
       float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), dc = ca;
+      int cq0 = 0, cq1 = 0, cq2 = 0, cq3 = 0;
       if (pi < W.end) {
-        uint32_t s = pairs[pi];
+        const uint32_t s = pairs[pi];
         SegGeom g;
         if (seg_geom(pos[s], v, g)) {
-          long maj0 = g.xmajor ? x0 : y0, min0 = g.xmajor ? y0 : x0;
+          const long maj0 = g.xmajor ? x0 : y0, min0 = g.xmajor ? y0 : x0;
           long a = g.klo > maj0 ? g.klo : maj0;
           long b = g.khi < maj0 + kTile - 1 ? g.khi : maj0 + kTile - 1;
           if (a <= b) minor_window(g, min0, min0 + kTile - 1, a, b);
           if (a <= b) {
-            n_samp = (int)(b - a + 1);
-            ka = (int)a;
-            xm = g.xmajor;
-            M0 = g.M0;
-            m0 = g.m0;
-            slope = g.slope;
-            inv_dM = g.inv_dM;
+            const double u0 = ((double)maj0 + 0.5) - g.M0;
+            m_anchor = (float)(g.m0 + u0 * g.slope - (double)min0);
+            slope = (float)g.slope;
             ca = col0[s];
-            float4 cb = col1 ? col1[s] : ca;
+            const float4 cb = col1 ? col1[s] : ca;
             dc = make_float4(cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w);
-            vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
+            const int vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
+            s_anchor = (float)(u0 * g.inv_dM);
+            ds = (float)g.inv_dM;
+            cq0 = fx(ca.x, cscale);
+            cq1 = fx(ca.y, cscale);
+            cq2 = fx(ca.z, cscale);
+            cq3 = fx(ca.w * ca.w, cscale);
+            packed = (int)(b - a + 1) | ((int)(a - maj0) << 8) | (g.xmajor << 16) | (vc << 17);
           }
         }
       }
       // ---- walk the pairs that have samples ----
-      uint64_t todo = __ballot(n_samp > 0);
+      uint64_t todo = __ballot(packed != 0);
       while (todo) {
         const int j = __builtin_ctzll(todo);
         todo &= todo - 1;
-        const int jn = bcast_i(n_samp, j);
-        const int jka = bcast_i(ka, j);
-        const int jxm = bcast_i(xm, j);
-        const int jvc = bcast_i(vc, j);
-        const double jM0 = bcast_d(M0, j), jm0 = bcast_d(m0, j);
-        const double jslope = bcast_d(slope, j), jinv = bcast_d(inv_dM, j);
-        const float cr = bcast_f(ca.x, j), cg = bcast_f(ca.y, j), cbl = bcast_f(ca.z, j),
-                    cal = bcast_f(ca.w, j);
-        const int maj0 = jxm ? x0 : y0, min0 = jxm ? y0 : x0;
-        int cq[4] = {fx(cr, cscale), fx(cg, cscale), fx(cbl, cscale), fx(cal * cal, cscale)};
-        float dr = 0.f, dg = 0.f, db = 0.f, da = 0.f;
-        if (jvc) {
-          dr = bcast_f(dc.x, j);
-          dg = bcast_f(dc.y, j);
-          db = bcast_f(dc.z, j);
-          da = bcast_f(dc.w, j);
-        }
-        for (int q = lane; q < jn; q += 64) {
-          const int k = jka + q;
-          const double u = ((double)k + 0.5) - jM0;
-          const double m = jm0 + u * jslope;
-          const double vv = m - 0.5;
-          const double jf = floor(vv);
-          const float f = (float)(vv - jf);
-          const int jl = (int)jf - min0;  // local minor index of the first deposit pixel
-          int wq = (int)(f * (float)kWOne + 0.5f);
+        const int jp = bcast_i(packed, j);
+        const int jn = jp & 0xff, jq0 = (jp >> 8) & 0xff, jxm = (jp >> 16) & 1, jvc = (jp >> 17) & 1;
+        const float jm = bcast_f(m_anchor, j), jsl = bcast_f(slope, j);
+        const long long c20_0 = (long long)bcast_i(cq0, j) << kWBits;
+        const long long c20_1 = (long long)bcast_i(cq1, j) << kWBits;
+        const long long c20_2 = (long long)bcast_i(cq2, j) << kWBits;
+        const long long c20_3 = (long long)bcast_i(cq3, j) << kWBits;
+        const int jcq0 = bcast_i(cq0, j), jcq1 = bcast_i(cq1, j), jcq2 = bcast_i(cq2, j),
+                  jcq3 = bcast_i(cq3, j);
+        for (int t = lane; t < jn; t += 64) {
+          const int q = jq0 + t;  // major index inside the tile
+          const float m = __builtin_fmaf((float)q, jsl, jm) - 0.5f;
+          const float jf = __builtin_floorf(m);
+          const float f = m - jf;
+          const int jl = (int)jf;  // local minor index of the first deposit pixel
+          int wq = (int)__builtin_fmaf(f, (float)kWOne, 0.5f);
           wq = wq > kWOne ? kWOne : wq;
-          const int w0 = kWOne - wq;
-          int c[4] = {cq[0], cq[1], cq[2], cq[3]};
-          if (jvc) {
-            const float sp = (float)(u * jinv);
-            const float al = cal + sp * da;
-            c[0] = fx(cr + sp * dr, cscale);
-            c[1] = fx(cg + sp * dg, cscale);
-            c[2] = fx(cbl + sp * db, cscale);
-            c[3] = fx(al * al, cscale);
+          long long hi0, hi1, hi2, hi3, lo0, lo1, lo2, lo3;
+          if (!jvc) {
+            hi0 = (long long)wq * jcq0;
+            hi1 = (long long)wq * jcq1;
+            hi2 = (long long)wq * jcq2;
+            hi3 = (long long)wq * jcq3;
+            lo0 = c20_0 - hi0;
+            lo1 = c20_1 - hi1;
+            lo2 = c20_2 - hi2;
+            lo3 = c20_3 - hi3;
+          } else {
+            const float sp = __builtin_fmaf((float)q, bcast_f(ds, j), bcast_f(s_anchor, j));
+            const float al = __builtin_fmaf(sp, bcast_f(dc.w, j), bcast_f(ca.w, j));
+            const int c0 = fx(__builtin_fmaf(sp, bcast_f(dc.x, j), bcast_f(ca.x, j)), cscale);
+            const int c1 = fx(__builtin_fmaf(sp, bcast_f(dc.y, j), bcast_f(ca.y, j)), cscale);
+            const int c2 = fx(__builtin_fmaf(sp, bcast_f(dc.z, j), bcast_f(ca.z, j)), cscale);
+            const int c3 = fx(al * al, cscale);
+            hi0 = (long long)wq * c0;
+            hi1 = (long long)wq * c1;
+            hi2 = (long long)wq * c2;
+            hi3 = (long long)wq * c3;
+            lo0 = ((long long)c0 << kWBits) - hi0;
+            lo1 = ((long long)c1 << kWBits) - hi1;
+            lo2 = ((long long)c2 << kWBits) - hi2;
+            lo3 = ((long long)c3 << kWBits) - hi3;
           }
-          const int kl = k - maj0;
+          // deposits: (1 - f) at minor jl, f at minor jl + 1
+          const int i0 = jxm ? (jl * kTileStride + q) : (q * kTileStride + jl);
+          const int step = jxm ? kTileStride : 1;
           if (jl >= 0 && jl < kTile) {
-            if (jxm)
-              lds_deposit(s_tile, jl, kl, w0, c);
-            else
-              lds_deposit(s_tile, kl, jl, w0, c);
+            atomicAdd(&s_tile[i0], (unsigned long long)lo0);
+            atomicAdd(&s_tile[kPlane + i0], (unsigned long long)lo1);
+            atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)lo2);
+            atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)lo3);
           }
           if (jl + 1 >= 0 && jl + 1 < kTile) {
-            if (jxm)
-              lds_deposit(s_tile, jl + 1, kl, wq, c);
-            else
-              lds_deposit(s_tile, kl, jl + 1, wq, c);
+            const int i1 = i0 + step;
+            atomicAdd(&s_tile[i1], (unsigned long long)hi0);
+            atomicAdd(&s_tile[kPlane + i1], (unsigned long long)hi1);
+            atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)hi2);
+            atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)hi3);
           }
         }
       }

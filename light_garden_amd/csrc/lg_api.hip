@@ -612,7 +612,8 @@ static int accumulate_impl(lg_ctx* ctx, const lg_segments* segs, lg_image* img) 
   AccumView v = make_view(img->view);
   const int n_tiles = v.tiles_x * v.tiles_y;
   if (n_tiles > 16384) return fail(LG_E_UNSUPPORTED, "accumulate: image above 8192x8192");
-  uint64_t chunk = 2048;
+  // about two chunks per CU: few enough that the [tile][chunk] count matrix stays small
+  uint64_t chunk = std::max<uint64_t>(1024, (n + 2 * (uint64_t)ctx->n_cu - 1) / (2 * (uint64_t)ctx->n_cu));
   while (((n + chunk - 1) / chunk) * (uint64_t)n_tiles > (64ull << 20)) chunk *= 2;
   uint32_t n_chunks = (uint32_t)((n + chunk - 1) / chunk);
   uint64_t n_cells = (uint64_t)n_tiles * n_chunks;
@@ -652,8 +653,8 @@ static int accumulate_impl(lg_ctx* ctx, const lg_segments* segs, lg_image* img) 
   }
   // work items: tiles split into pieces of at most `piece` pairs, largest first
   const int grid = ctx->n_cu;
-  const uint64_t kMaxPiece = 16384;
-  uint64_t piece = std::min<uint64_t>(kMaxPiece, std::max<uint64_t>(512, total / ((uint64_t)grid * 4) + 1));
+  const uint64_t kMaxPiece = 1u << 20;
+  uint64_t piece = std::min<uint64_t>(kMaxPiece, std::max<uint64_t>(4096, total / ((uint64_t)grid * 2) + 1));
   // fixed point of the LDS tile: colour channel c -> round(c * 2^G), weights in 2^-20. A
   // pixel takes at most one deposit per pair, so an item of <= kMaxPiece pairs stays below
   // kMaxPiece * 2^20 * 2^31 < 2^63 when |c| * 2^G < 2^31.

@@ -412,17 +412,16 @@ class Tracer:
 
     def trace_all(self):
         """Tracer::trace_all (tracer.rs:276): vertex pairs in the reference's order, as
-        (xy float64 (2S, 2), rgba float32 (2S, 4))."""
-        self.sync()
-        p = self.params()
-        n = C.c_uint64()
-        rc = abi.load().lg_trace_all(self._h, C.byref(p), None, None, 0, C.byref(n))
-        if rc not in (abi.LG_OK, abi.LG_E_CAPACITY):
-            abi.check(rc, "lg_trace_all")
-        xy = np.empty((n.value, 2), np.float64)
-        rgba = np.empty((n.value, 4), np.float32)
-        abi.call("lg_trace_all", self._h, C.byref(p), xy.ctypes.data, rgba.ctypes.data, n.value,
-                 C.byref(n))
+        (xy float64 (2S, 2), rgba float32 (2S, 4)). One trace; the segment keys give the
+        order (light, ray, depth, parent, reflected before refracted)."""
+        segs = self.trace()
+        pos, col, _, key = segs.download()
+        if self.fp64:
+            pos = segs.download_pos64()
+        segs.close()
+        order = np.argsort(key, kind="stable")
+        xy = np.asarray(pos, np.float64)[order].reshape(-1, 2)
+        rgba = np.repeat(col[order], 2, axis=0)
         return xy, rgba
 
     def render(self, segs: Segments, img: Image):

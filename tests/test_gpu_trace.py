@@ -177,3 +177,12 @@ def test_trace_shape_zoo(ctx, orc, fp64):
     assert m["matched"] >= 0.999 * m["n_b"]
     assert m["diverged"] <= 0.001 * m["n_b"]
     assert m["max_err"] < (1e-9 if fp64 else 1e-6)
+
+
+def test_python_trace_all_matches_c_entry(ctx, orc):
+    scene = configs.plain_small(num_rays=700)
+    tr = make_tracer(ctx, scene, fp64=True)
+    xy, rgba = tr.trace_all()
+    rpos, rcol, _ = orc.trace_scene(scene)
+    np.testing.assert_allclose(xy, rpos.reshape(-1, 2), atol=1e-9, rtol=0)
+    np.testing.assert_array_equal(rgba, np.repeat(rcol, 2, axis=0))
