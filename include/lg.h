@@ -229,6 +229,28 @@ int lg_resolve(lg_ctx* ctx, lg_image* img);
  * mod.rs:680-696, renderer.rs:420). `segs` receives the trace output. */
 int lg_render(lg_scene* scene, const lg_params* params, lg_segments* segs, lg_image* img);
 
+/* ---- batched form (no reference counterpart: the reference renders one scene per frame) --
+ * lg_render for n scenes in one frame: every stage runs once over all scenes (persistent
+ * work-queue kernels over device-built work lists) and the launch sequence is replayed from a
+ * hipGraph while the batch keeps its shape. Scenes, segment buffers and images must be
+ * distinct; all params must agree on LG_TRACE_FP64. Results equal n separate lg_render
+ * calls bit for bit (accumulation sums are exact integers until one final conversion). */
+int lg_batch_render(lg_ctx* ctx, uint32_t n, lg_scene* const* scenes, const lg_params* params,
+                    lg_segments* const* segs, lg_image* const* images);
+/* Launch tuning of a context. No counterpart in the reference (its trace is one CPU pass):
+ * these are the knobs of the persistent GPU work lists, read by the autotuner. None of them
+ * changes results: images are the same bits for every setting. */
+typedef struct lg_tuning {
+  uint32_t rays_per_item;  /* rays per trace work item, [64, 2^20]; default 256 */
+  uint32_t min_piece;      /* smallest tile work item in segment-tile pairs, >= 64; default 4096 */
+  uint32_t max_piece;      /* largest tile work item, >= min_piece; default 8192 */
+  uint32_t trace_waves;    /* trace workgroups per compute unit, [1, 16]; default 4 */
+} lg_tuning;
+int lg_ctx_get_tuning(const lg_ctx* ctx, lg_tuning* out);
+int lg_ctx_set_tuning(lg_ctx* ctx, const lg_tuning* tuning);
+/* Replay frames from a hipGraph while the batch keeps its shape (default on). */
+int lg_ctx_set_graphs(lg_ctx* ctx, int enable);
+
 #ifdef __cplusplus
 }
 #endif

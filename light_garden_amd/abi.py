@@ -87,6 +87,11 @@ class LgStageTime(C.Structure):
                 ("work", C.c_uint64)]
 
 
+class LgTuning(C.Structure):
+    _fields_ = [("rays_per_item", C.c_uint32), ("min_piece", C.c_uint32), ("max_piece", C.c_uint32),
+                ("trace_waves", C.c_uint32)]
+
+
 assert C.sizeof(LgShape) == 136
 assert C.sizeof(LgObject) == 16 + 7 * 136 + 256
 assert C.sizeof(LgLight) == 72
@@ -135,6 +140,11 @@ FUNCTIONS = {
     "lg_accumulate": (C.c_int, [_P, _P, _P]),
     "lg_resolve": (C.c_int, [_P, _P]),
     "lg_render": (C.c_int, [_P, C.POINTER(LgParams), _P, _P]),
+    "lg_batch_render": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(LgParams),
+                                  C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "lg_ctx_set_graphs": (C.c_int, [_P, C.c_int]),
+    "lg_ctx_get_tuning": (C.c_int, [_P, C.POINTER(LgTuning)]),
+    "lg_ctx_set_tuning": (C.c_int, [_P, C.POINTER(LgTuning)]),
 }
 
 _lib = None

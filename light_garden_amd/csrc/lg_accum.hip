@@ -123,70 +123,243 @@ __device__ __forceinline__ void for_each_tile(const SegGeom& g, const AccumView&
   }
 }
 
-// ---- binning -----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict__ pos, uint64_t n,
-                                                        AccumView v, uint32_t chunk,
-                                                        uint32_t n_chunks, uint32_t* counts) {
-  extern __shared__ uint32_t s_hist[];
-  const int n_tiles = v.tiles_x * v.tiles_y;
-  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x) s_hist[t] = 0;
-  __syncthreads();
-  uint64_t s0 = (uint64_t)blockIdx.x * chunk;
-  uint64_t s1 = s0 + chunk < n ? s0 + chunk : n;
-  for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-    SegGeom g;
-    if (!seg_geom(pos[s], v, g)) continue;
-    for_each_tile(g, v, [&](int tile) { atomicAdd(&s_hist[tile], 1u); });
+// ---- batch helpers ------------------------------------------------------------------------
+__device__ __forceinline__ int find_u32(const uint32_t* pre, int n, uint32_t x) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pre[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
-    counts[(uint64_t)t * n_chunks + blockIdx.x] = s_hist[t];
+  return lo;
 }
 
-__global__ __launch_bounds__(256) void bin_fill_kernel(const float4* __restrict__ pos, uint64_t n,
-                                                       AccumView v, uint32_t chunk,
-                                                       uint32_t n_chunks,
-                                                       const uint32_t* __restrict__ offsets,
-                                                       uint32_t* pairs) {
-  extern __shared__ uint32_t s_cur[];
-  const int n_tiles = v.tiles_x * v.tiles_y;
-  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
-    s_cur[t] = offsets[(uint64_t)t * n_chunks + blockIdx.x];
-  __syncthreads();
-  uint64_t s0 = (uint64_t)blockIdx.x * chunk;
-  uint64_t s1 = s0 + chunk < n ? s0 + chunk : n;
-  for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-    SegGeom g;
-    if (!seg_geom(pos[s], v, g)) continue;
-    for_each_tile(g, v, [&](int tile) {
-      uint32_t slot = atomicAdd(&s_cur[tile], 1u);
-      pairs[slot] = (uint32_t)s;
-    });
+__device__ __forceinline__ int find_u64(const uint64_t* pre, int n, uint64_t x) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pre[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
   }
+  return lo;
 }
 
-// tile_start[t] = offsets[t * n_chunks], tile_start[n_tiles] = low 32 bits of the total
-__global__ void tile_starts_kernel(const uint32_t* offsets, uint32_t n_chunks, int n_tiles,
-                                   const unsigned long long* total64, uint32_t* tile_start) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n_tiles) tile_start[t] = offsets[(uint64_t)t * n_chunks];
-  if (t == n_tiles) tile_start[t] = (uint32_t)*total64;
+__device__ __forceinline__ uint64_t scene_segments(const SceneDesc& D) {
+  if (D.flags & SCENE_FIXED_COUNT) return D.seg_fixed;
+  uint64_t c = *D.seg_count;
+  return c < D.seg_cap ? c : D.seg_cap;
 }
 
-// total64 += sum(in[0..n)) in 64 bits (total64 zeroed by the caller)
-__global__ __launch_bounds__(256) void sum_u64_kernel(const uint32_t* in, uint64_t n,
-                                                      unsigned long long* total64) {
-  __shared__ unsigned long long s_w[4];
-  unsigned long long acc = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    acc += in[i];
-  for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+// Bin layout of one frame, from the segment counts the trace just produced: the chunk size
+// gives about `target` chunks over the whole batch (at least `chunk_min` segments each);
+// chunk_pre[s] = first chunk of scene s, cbase[s] = first entry of scene s's [tile][chunk]
+// count matrix (tiles x its chunk count). Resets the accumulation part of the control block.
+// One block of 1024.
+__global__ __launch_bounds__(1024) void build_chunks_kernel(const SceneDesc* __restrict__ descs, int n_scenes,
+                                                           uint32_t chunk_min, uint32_t target,
+                                                           uint32_t* chunk_pre, uint32_t* cbase,
+                                                           BatchCtl* ctl) {
+  __shared__ unsigned long long s_tot[16];
+  __shared__ uint32_t s_wa[16], s_wb[16];
+  __shared__ uint32_t s_ca, s_cb;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // total segments of the batch
+  unsigned long long tot = 0;
+  for (int sid = threadIdx.x; sid < n_scenes; sid += 1024) tot += scene_segments(descs[sid]);
+  for (int d = 32; d > 0; d >>= 1) tot += __shfl_down(tot, d, 64);
+  if (lane == 0) s_tot[wave] = tot;
+  if (threadIdx.x == 0) s_ca = s_cb = 0;
   __syncthreads();
+  unsigned long long total = 0;
+  for (int w = 0; w < 16; w++) total += s_tot[w];
+  unsigned long long ch = (total + target - 1) / target;
+  const uint32_t chunk = (uint32_t)(ch > chunk_min ? (ch < 0x80000000ull ? ch : 0x80000000ull) : chunk_min);
+  for (int base = 0; base < n_scenes; base += 1024) {
+    const int sid = base + threadIdx.x;
+    uint32_t nch = 0, ncnt = 0;
+    if (sid < n_scenes) {
+      const SceneDesc& D = descs[sid];
+      nch = (uint32_t)((scene_segments(D) + chunk - 1) / chunk);
+      ncnt = nch * (uint32_t)(D.view.tiles_x * D.view.tiles_y);
+    }
+    uint32_t xa = nch, xb = ncnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t ya = __shfl_up(xa, d, 64), yb = __shfl_up(xb, d, 64);
+      if (lane >= d) {
+        xa += ya;
+        xb += yb;
+      }
+    }
+    if (lane == 63) {
+      s_wa[wave] = xa;
+      s_wb[wave] = xb;
+    }
+    __syncthreads();
+    uint32_t oa = s_ca, ob = s_cb;
+    for (int w = 0; w < wave; w++) {
+      oa += s_wa[w];
+      ob += s_wb[w];
+    }
+    if (sid < n_scenes) {
+      chunk_pre[sid] = oa + xa - nch;
+      cbase[sid] = ob + xb - ncnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) {
+      s_ca = oa + xa;
+      s_cb = ob + xb;
+    }
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
-    unsigned long long t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-    if (t) atomicAdd(total64, t);
+    chunk_pre[n_scenes] = s_ca;
+    cbase[n_scenes] = s_cb;
+    ctl->chunk = chunk;
+    ctl->n_chunks = s_ca;
+    ctl->n_counts = s_cb;
+    ctl->chunk_cursor[0] = 0;
+    ctl->chunk_cursor[1] = 0;
+    ctl->total_pairs = 0;
+    ctl->n_work = 0;
+    ctl->work_cursor = 0;
+  }
+}
+
+// Persistent over bin chunks. PASS 0 counts pairs per tile into the scene's [tile][chunk]
+// matrix; PASS 1 writes segment indices at the scanned offsets.
+template <int PASS>
+__global__ __launch_bounds__(256) void bin_kernel(const SceneDesc* __restrict__ descs, int n_scenes,
+                                                  const uint32_t* __restrict__ chunk_pre,
+                                                  const uint32_t* __restrict__ cbase, BatchCtl* ctl,
+                                                  uint32_t* counts_or_offsets, uint32_t* pairs,
+                                                  uint64_t pairs_cap) {
+  extern __shared__ uint32_t s_hist[];
+  __shared__ uint32_t s_chunk;
+  const uint32_t n_chunks = ctl->n_chunks;
+  const uint32_t chunk = ctl->chunk;
+  for (;;) {
+    if (threadIdx.x == 0) s_chunk = atomicAdd(&ctl->chunk_cursor[PASS], 1u);
+    __syncthreads();
+    const uint32_t ci = __builtin_amdgcn_readfirstlane(s_chunk);
+    if (ci >= n_chunks) break;
+    const int sid = __builtin_amdgcn_readfirstlane(find_u32(chunk_pre, n_scenes, ci));
+    const SceneDesc& D = descs[sid];
+    const AccumView v = D.view;
+    const int n_tiles = v.tiles_x * v.tiles_y;
+    const uint32_t local = ci - chunk_pre[sid];
+    const uint32_t nch = chunk_pre[sid + 1] - chunk_pre[sid];
+    uint32_t* mat = counts_or_offsets + cbase[sid];
+    for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
+      s_hist[t] = PASS == 0 ? 0u : mat[(uint64_t)t * nch + local];
+    __syncthreads();
+    const uint64_t n = scene_segments(D);
+    const uint64_t s0 = (uint64_t)local * chunk;
+    const uint64_t s1 = s0 + chunk < n ? s0 + chunk : n;
+    for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+      SegGeom g;
+      if (!seg_geom(gen(D.seg_pos)[s], v, g)) continue;
+      if (PASS == 0) {
+        for_each_tile(g, v, [&](int tile) { atomicAdd(&s_hist[tile], 1u); });
+      } else {
+        for_each_tile(g, v, [&](int tile) {
+          uint32_t slot = atomicAdd(&s_hist[tile], 1u);
+          if (slot < pairs_cap) pairs[slot] = (uint32_t)s;
+        });
+      }
+    }
+    __syncthreads();
+    if (PASS == 0)
+      for (int t = threadIdx.x; t < n_tiles; t += blockDim.x) mat[(uint64_t)t * nch + local] = s_hist[t];
+    __syncthreads();
+  }
+}
+
+// tile_start[g] for every global tile; tile_start[T] = total pairs; sets the work-item size.
+__global__ void tile_starts_kernel(const SceneDesc* __restrict__ descs, const uint32_t* __restrict__ tile_scene,
+                                   int n_tiles_total, const uint32_t* __restrict__ chunk_pre,
+                                   const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ offsets,
+                                   BatchCtl* ctl, uint32_t* tile_start, uint64_t pairs_cap, uint32_t min_piece,
+                                   uint32_t max_piece, uint32_t grid) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n_tiles_total) {
+    const uint32_t sid = tile_scene[g];
+    const int t = g - descs[sid].tile_base;
+    const uint32_t nch = chunk_pre[sid + 1] - chunk_pre[sid];
+    // a scene without chunks owns no entries: its tiles start where the next scene does
+    tile_start[g] = offsets[cbase[sid] + (uint64_t)t * nch];
+  }
+  if (g == n_tiles_total) {
+    const unsigned long long total = ctl->total_pairs;
+    tile_start[g] = (uint32_t)total;
+    if (total > pairs_cap) atomicOr(&ctl->flags, (uint32_t)CTL_PAIRS_OVERFLOW);
+    unsigned long long p = total / (2ull * grid) + 1;
+    p = p < min_piece ? min_piece : p;
+    p = p > max_piece ? max_piece : p;
+    ctl->piece = (uint32_t)p;
+  }
+}
+
+// pieces[g] = number of work items of tile g; counts the items of every size class.
+__global__ void tile_pieces_kernel(const uint32_t* __restrict__ tile_start, int n_tiles_total, BatchCtl* ctl,
+                                   uint32_t* pieces) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_tiles_total) return;
+  const uint32_t piece = ctl->piece;
+  const uint32_t cnt = tile_start[g + 1] - tile_start[g];
+  const uint32_t full = cnt / piece, rem = cnt - full * piece;
+  pieces[g] = full + (rem ? 1u : 0u);
+  if (full) atomicAdd(&ctl->class_count[31 - __clz(piece)], full);
+  if (rem) atomicAdd(&ctl->class_count[31 - __clz(rem)], 1u);
+}
+
+// First slot of every size class, largest class first; the number of work items. One wave.
+__global__ void class_bases_kernel(BatchCtl* ctl, uint32_t items_cap) {
+  if (threadIdx.x != 0) return;
+  uint32_t base = 0;
+  for (int c = 31; c >= 0; c--) {
+    ctl->class_base[c] = base;
+    base += ctl->class_count[c];
+  }
+  if (base > items_cap) atomicOr(&ctl->flags, (uint32_t)CTL_ITEMS_OVERFLOW);
+  ctl->n_work = base < items_cap ? base : items_cap;
+}
+
+// Writes the work items of every tile into the slots of their size classes.
+__global__ void write_items_kernel(const SceneDesc* __restrict__ descs, const uint32_t* __restrict__ tile_scene,
+                                   const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ pieces,
+                                   int n_tiles_total, BatchCtl* ctl, WorkItem* items, uint32_t items_cap,
+                                   uint32_t* tile_split, uint32_t* split_tiles, uint32_t split_cap) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_tiles_total) return;
+  const uint32_t piece = ctl->piece;
+  const uint32_t np = pieces[g];
+  if (np == 0) return;
+  const uint32_t b = tile_start[g], e = tile_start[g + 1];
+  const uint32_t sid = tile_scene[g];
+  const uint32_t t = (uint32_t)(g - descs[sid].tile_base);
+  if (np > 1) {  // the tile's items merge their sums in a slot of the fixed-point merge buffer
+    const uint32_t slot = atomicAdd(&ctl->n_split, 1u);
+    if (slot < split_cap) {
+      split_tiles[slot] = (uint32_t)g;
+      tile_split[g] = slot;
+    } else {
+      atomicOr(&ctl->flags, (uint32_t)CTL_SPLIT_OVERFLOW);
+    }
+  }
+  for (uint32_t p = 0; p < np; p++) {
+    WorkItem w;
+    w.scene = sid | (np > 1 ? kSplitBit : 0u);
+    w.tile = t;
+    w.begin = b + p * piece;
+    w.end = (w.begin + piece < e) ? w.begin + piece : e;
+    const int cls = 31 - __clz(w.end - w.begin);
+    const uint32_t slot = ctl->class_base[cls] + atomicAdd(&ctl->class_cursor[cls], 1u);
+    if (slot < items_cap) items[slot] = w;
   }
 }
 
@@ -240,15 +413,51 @@ __global__ __launch_bounds__(kScanThreads) void scan_add_kernel(uint32_t* out, u
     if (base + i < n) out[base + i] += add;
 }
 
+// ctl->total_pairs += sum(in[0..n)) in 64 bits
+__global__ __launch_bounds__(256) void sum_u64_kernel(const uint32_t* in, uint64_t n, BatchCtl* ctl) {
+  __shared__ unsigned long long s_w[4];
+  unsigned long long acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    acc += in[i];
+  for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    if (t) atomicAdd(&ctl->total_pairs, t);
+  }
+}
+
 // ---- tile accumulation -----------------------------------------------------------------
 // The LDS tile accumulates in 64-bit fixed point: FP32 LDS atomics (ds_add_f32) run ~30x
 // slower than 64-bit integer ones (ds_add_u64) on gfx950. A sample's anti-aliasing weight f
 // is quantised to wq = round(f * 2^20) and the pixel pair gets (2^20 - wq) * cq and wq * cq,
 // where cq = round(c * 2^G) is the channel value in fixed point: the two deposits sum to
-// exactly 2^20 * cq. G is chosen per call from the largest |colour| so that no tile can
-// overflow 63 bits.
+// exactly 2^20 * cq. Integer sums do not depend on the order of the deposits, so a frame's
+// image is the same bits whatever the batch, the work split or the replay: a tile whose
+// pairs are split over several work items adds its exact sums into a 64-bit merge slot, and
+// every pixel's sum is converted to FP32 once, by the same expression, whether the tile was
+// split or not.
 constexpr int kWBits = 20;
 constexpr int kWOne = 1 << kWBits;
+constexpr int kSlotU64 = 4 * kTile * kTile;  // one merge slot: 4 channel planes of a tile
+
+// G of a scene's fixed point, from the scene's colour bound and its segment count only (never
+// from a buffer capacity, so it is the same alone or in any batch). A segment deposits at most
+// once into a given pixel, so every partial sum of a pixel is bounded by
+// n * 2^20 * max|cq|; with n <= 2^lb and G + log2(bound) <= 41 - lb that stays below 2^62,
+// and cq below 2^30.
+__device__ __forceinline__ int scene_fixed_bits(const SceneDesc& D) {
+  const double m = (double)D.color_max;
+  const double bound = m > m * m ? m : m * m;
+  if (!(bound > 0.0) || !(bound < 1e300)) return 30;
+  const uint64_t n = scene_segments(D);
+  const int lb = n > 1 ? 64 - __clzll((long long)(n - 1)) : 0;  // n <= 2^lb
+  const double room = (42 - lb) < 30 ? (double)(42 - lb) : 30.0;
+  const int G = (int)floor(room - log2(bound)) - 1;
+  return G < -30 ? -30 : (G > 60 ? 60 : G);
+}
 
 __device__ __forceinline__ void lds_deposit(unsigned long long* tile, int row, int col, int w,
                                             const int cq[4]) {
@@ -284,24 +493,34 @@ __device__ __forceinline__ int fx(float v, float scale) { return (int)__builtin_
 // q = k - maj0, in FP32 relative to the tile's minor origin. Two tiles sharing a sample
 // differ only by the FP32 rounding of their anchors (< 1e-5 px).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void tile_accum_kernel(
-    const float4* __restrict__ pos, const float4* __restrict__ col0,
-    const float4* __restrict__ col1, const uint32_t* __restrict__ pairs,
-    const WorkItem* __restrict__ items, uint32_t n_items, uint32_t* item_counter, AccumView v,
-    float4* accum, int G) {
+__global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __restrict__ descs,
+                                                          const uint32_t* __restrict__ pairs,
+                                                          const WorkItem* __restrict__ items,
+                                                          const uint32_t* __restrict__ tile_split,
+                                                          unsigned long long* split_fx, BatchCtl* ctl) {
   __shared__ unsigned long long s_tile[4 * kPlane];
   __shared__ uint32_t s_item;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   constexpr int kWaves = BLOCK / 64;
-  const float cscale = __builtin_ldexpf(1.0f, G);
-  const double unscale = __builtin_ldexp(1.0, -(G + kWBits));
+  if (ctl->flags & kCtlStop) return;  // a buffer overflowed: the host reruns the frame
+  const uint32_t n_items = ctl->n_work;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(item_counter, 1u);
+    if (threadIdx.x == 0) s_item = atomicAdd(&ctl->work_cursor, 1u);
     __syncthreads();
-    uint32_t it = s_item;
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= n_items) break;
     const WorkItem W = items[it];
+    const bool split = (W.scene & kSplitBit) != 0;
+    const SceneDesc& D = descs[W.scene & ~kSplitBit];
+    const AccumView v = D.view;
+    const float4* __restrict__ pos = gen(D.seg_pos);
+    const float4* __restrict__ col0 = gen(D.seg_col);
+    const float4* __restrict__ col1 = gen(D.seg_col1);
+    float4* accum = gen(D.accum);
+    const int G = scene_fixed_bits(D);
+    const float cscale = __builtin_ldexpf(1.0f, G);
+    const double unscale = __builtin_ldexp(1.0, -(G + kWBits));
     for (int k = threadIdx.x; k < 4 * kPlane; k += BLOCK) s_tile[k] = 0ull;
     __syncthreads();
     const int tx = (int)(W.tile % (uint32_t)v.tiles_x);
@@ -408,28 +627,30 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(
       }
     }
     __syncthreads();
-    // flush: fixed point -> FP32 accumulator
-    for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
-      int r = k / kTile, cc = k % kTile;
-      int gx = x0 + cc, gy = y0 + r;
-      if (gx >= v.width || gy >= v.height) continue;
-      int idx = r * kTileStride + cc;
-      float4 d = make_float4((float)((double)(long long)s_tile[idx] * unscale),
-                             (float)((double)(long long)s_tile[kPlane + idx] * unscale),
-                             (float)((double)(long long)s_tile[2 * kPlane + idx] * unscale),
-                             (float)((double)(long long)s_tile[3 * kPlane + idx] * unscale));
-      float4* dst = accum + (size_t)gy * v.width + gx;
-      if (W.split) {
-        if (d.x != 0.0f) atomicAdd(&dst->x, d.x);
-        if (d.y != 0.0f) atomicAdd(&dst->y, d.y);
-        if (d.z != 0.0f) atomicAdd(&dst->z, d.z);
-        if (d.w != 0.0f) atomicAdd(&dst->w, d.w);
-      } else {
+    if (split) {
+      // exact partial sums into the tile's merge slot (integer atomics: order independent)
+      unsigned long long* fx = split_fx + (size_t)tile_split[D.tile_base + W.tile] * kSlotU64;
+      for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
+        const int idx = (k / kTile) * kTileStride + (k % kTile);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const unsigned long long val = s_tile[c * kPlane + idx];
+          if (val) atomicAdd(&fx[c * kTile * kTile + k], val);
+        }
+      }
+    } else {
+      // the only item of its tile: fixed point -> FP32 accumulator
+      for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
+        int r = k / kTile, cc = k % kTile;
+        int gx = x0 + cc, gy = y0 + r;
+        if (gx >= v.width || gy >= v.height) continue;
+        int idx = r * kTileStride + cc;
+        float4* dst = accum + (size_t)gy * v.width + gx;
         float4 o = *dst;
-        o.x += d.x;
-        o.y += d.y;
-        o.z += d.z;
-        o.w += d.w;
+        o.x += (float)((double)(long long)s_tile[idx] * unscale);
+        o.y += (float)((double)(long long)s_tile[kPlane + idx] * unscale);
+        o.z += (float)((double)(long long)s_tile[2 * kPlane + idx] * unscale);
+        o.w += (float)((double)(long long)s_tile[3 * kPlane + idx] * unscale);
         *dst = o;
       }
     }
@@ -437,63 +658,79 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(
   }
 }
 
-// ---- resolve / clear ----------------------------------------------------------------------
-__global__ void resolve_kernel(const float4* __restrict__ accum, ushort4* __restrict__ out,
-                               uint64_t n) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float4 a = accum[i];
-  _Float16 r = (_Float16)a.x, g = (_Float16)a.y, b = (_Float16)a.z, al = (_Float16)(a.w + 1.0f);
-  ushort4 o;
-  o.x = __builtin_bit_cast(unsigned short, r);
-  o.y = __builtin_bit_cast(unsigned short, g);
-  o.z = __builtin_bit_cast(unsigned short, b);
-  o.w = __builtin_bit_cast(unsigned short, al);
-  out[i] = o;
+// Converts the merge slots of split tiles into the FP32 accumulator (the same expression as
+// the unsplit flush) and leaves the slots zeroed for the next frame. Persistent over slots.
+__global__ __launch_bounds__(256) void tile_merge_kernel(const SceneDesc* __restrict__ descs,
+                                                        const uint32_t* __restrict__ tile_scene,
+                                                        const uint32_t* __restrict__ split_tiles,
+                                                        const BatchCtl* ctl, unsigned long long* split_fx,
+                                                        uint32_t split_cap) {
+  if (ctl->flags & kCtlStop) return;
+  uint32_t n_split = ctl->n_split;
+  n_split = n_split < split_cap ? n_split : split_cap;
+  for (uint32_t sl = blockIdx.x; sl < n_split; sl += gridDim.x) {
+    const uint32_t g = split_tiles[sl];
+    const SceneDesc& D = descs[tile_scene[g]];
+    const AccumView v = D.view;
+    const uint32_t t = g - (uint32_t)D.tile_base;
+    const int x0 = (int)(t % (uint32_t)v.tiles_x) * kTile, y0 = (int)(t / (uint32_t)v.tiles_x) * kTile;
+    const double unscale = __builtin_ldexp(1.0, -(scene_fixed_bits(D) + kWBits));
+    unsigned long long* fx = split_fx + (size_t)sl * kSlotU64;
+    for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
+      unsigned long long a[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) a[c] = fx[c * kTile * kTile + k];
+      if ((a[0] | a[1] | a[2] | a[3]) == 0ull) continue;
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (a[c]) fx[c * kTile * kTile + k] = 0ull;
+      const int gx = x0 + (k % kTile), gy = y0 + (k / kTile);
+      if (gx >= v.width || gy >= v.height) continue;
+      float4* dst = gen(D.accum) + (size_t)gy * v.width + gx;
+      float4 o = *dst;
+      o.x += (float)((double)(long long)a[0] * unscale);
+      o.y += (float)((double)(long long)a[1] * unscale);
+      o.z += (float)((double)(long long)a[2] * unscale);
+      o.w += (float)((double)(long long)a[3] * unscale);
+      *dst = o;
+    }
+  }
 }
 
-__global__ void fill_zero_kernel(float4* p, uint64_t n) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+// ---- resolve / clear over every scene of a batch ------------------------------------------
+// px_pre[s] = first pixel of scene s in the batch's concatenated pixel range.
+template <int MODE>  // 0 clear, 1 resolve
+__global__ void pixels_kernel(const SceneDesc* __restrict__ descs, const uint64_t* __restrict__ px_pre,
+                              int n_scenes, uint64_t n_px) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_px) return;
+  const int sid = find_u64(px_pre, n_scenes, i);
+  const SceneDesc& D = descs[sid];
+  const uint64_t p = i - px_pre[sid];
+  if (MODE == 0) {
+    gen(D.accum)[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
+    const float4 a = gen(D.accum)[p];
+    const _Float16 r = (_Float16)a.x, g = (_Float16)a.y, b = (_Float16)a.z, al = (_Float16)(a.w + 1.0f);
+    ushort4 o;
+    o.x = __builtin_bit_cast(unsigned short, r);
+    o.y = __builtin_bit_cast(unsigned short, g);
+    o.z = __builtin_bit_cast(unsigned short, b);
+    o.w = __builtin_bit_cast(unsigned short, al);
+    gen(D.out16)[p] = o;
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // ---- host launchers ---------------------------------------------------------------------
 int tile_size() { return kTile; }
-
-hipError_t launch_bin_count(const float4* pos, uint64_t n, const AccumView& v, uint32_t chunk,
-                            uint32_t n_chunks, uint32_t* counts, hipStream_t st) {
-  size_t lds = (size_t)v.tiles_x * v.tiles_y * sizeof(uint32_t);
-  hipLaunchKernelGGL(bin_count_kernel, dim3(n_chunks), dim3(256), lds, st, pos, n, v, chunk,
-                     n_chunks, counts);
-  return hipGetLastError();
-}
-
-hipError_t launch_bin_fill(const float4* pos, uint64_t n, const AccumView& v, uint32_t chunk,
-                           uint32_t n_chunks, const uint32_t* offsets, uint32_t* pairs,
-                           hipStream_t st) {
-  size_t lds = (size_t)v.tiles_x * v.tiles_y * sizeof(uint32_t);
-  hipLaunchKernelGGL(bin_fill_kernel, dim3(n_chunks), dim3(256), lds, st, pos, n, v, chunk,
-                     n_chunks, offsets, pairs);
-  return hipGetLastError();
-}
-
-hipError_t launch_tile_starts(const uint32_t* offsets, uint32_t n_chunks, int n_tiles,
-                              const unsigned long long* total64, uint32_t* tile_start,
-                              hipStream_t st) {
-  int n = n_tiles + 1;
-  hipLaunchKernelGGL(tile_starts_kernel, dim3((n + 255) / 256), dim3(256), 0, st, offsets,
-                     n_chunks, n_tiles, total64, tile_start);
-  return hipGetLastError();
-}
-
-hipError_t launch_sum_u64(const uint32_t* in, uint64_t n, unsigned long long* total64,
-                          hipStream_t st) {
-  uint64_t blocks = (n + 255) / 256;
-  if (blocks > 1024) blocks = 1024;
-  if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(sum_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, n, total64);
-  return hipGetLastError();
-}
+int tile_block() { return kTileBlock; }
+int weight_bits() { return kWBits; }
+uint64_t merge_slot_u64() { return kSlotU64; }
 
 uint64_t scan_scratch_elems(uint64_t n) {
   uint64_t total = 0;
@@ -505,15 +742,13 @@ uint64_t scan_scratch_elems(uint64_t n) {
   return total + 1;
 }
 
-// Exclusive scan in place over out (in may equal out). scratch holds
-// scan_scratch_elems(n) u32.
+// Exclusive scan of n u32 (out may not alias in). scratch holds scan_scratch_elems(n) u32.
 hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* scratch,
                                  hipStream_t st) {
   if (n == 0) return hipSuccess;
   uint64_t nb = (n + kScanBlock - 1) / kScanBlock;
   uint32_t* sums = scratch;
-  hipLaunchKernelGGL(scan_block_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n,
-                     sums);
+  hipLaunchKernelGGL(scan_block_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nb == 1) return e;
   e = launch_scan_exclusive(sums, sums, nb, scratch + nb, st);
@@ -522,26 +757,87 @@ hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, 
   return hipGetLastError();
 }
 
-hipError_t launch_tile_accum(const float4* pos, const float4* col0, const float4* col1,
-                             const uint32_t* pairs, const WorkItem* items, uint32_t n_items,
-                             uint32_t* item_counter, const AccumView& v, float4* accum, int grid,
-                             int fixed_bits, hipStream_t st) {
-  hipLaunchKernelGGL((tile_accum_kernel<kTileBlock>), dim3(grid), dim3(kTileBlock), 0, st, pos, col0,
-                     col1, pairs, items, n_items, item_counter, v, accum, fixed_bits);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
 
-int tile_block() { return kTileBlock; }
-int weight_bits() { return kWBits; }
-
-hipError_t launch_resolve(const float4* accum, ushort4* out, uint64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(resolve_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, accum,
-                     out, n);
+hipError_t launch_build_chunks(const SceneDesc* descs, int n_scenes, uint32_t chunk_min, uint32_t target,
+                               uint32_t* chunk_pre, uint32_t* cbase, BatchCtl* ctl, hipStream_t st) {
+  hipLaunchKernelGGL(build_chunks_kernel, dim3(1), dim3(1024), 0, st, descs, n_scenes, chunk_min, target, chunk_pre,
+                     cbase, ctl);
   return hipGetLastError();
 }
 
-hipError_t launch_fill_zero(float4* p, uint64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+hipError_t launch_bin(int pass, const SceneDesc* descs, int n_scenes, const uint32_t* chunk_pre,
+                      const uint32_t* cbase, BatchCtl* ctl, uint32_t* mat, uint32_t* pairs, uint64_t pairs_cap,
+                      int max_tiles, int grid, hipStream_t st) {
+  size_t lds = (size_t)max_tiles * sizeof(uint32_t);
+  if (pass == 0)
+    hipLaunchKernelGGL(bin_kernel<0>, dim3(grid), dim3(256), lds, st, descs, n_scenes, chunk_pre, cbase, ctl, mat,
+                       pairs, pairs_cap);
+  else
+    hipLaunchKernelGGL(bin_kernel<1>, dim3(grid), dim3(256), lds, st, descs, n_scenes, chunk_pre, cbase, ctl, mat,
+                       pairs, pairs_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_u64(const uint32_t* in, uint64_t n, BatchCtl* ctl, hipStream_t st) {
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(sum_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, n, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_work_items(const SceneDesc* descs, const uint32_t* tile_scene, int n_tiles_total,
+                             const uint32_t* chunk_pre, const uint32_t* cbase, const uint32_t* offsets,
+                             BatchCtl* ctl, uint32_t* tile_start, uint32_t* pieces,
+                             WorkItem* items, uint32_t items_cap, uint32_t* tile_split, uint32_t* split_tiles,
+                             uint32_t split_cap, uint64_t pairs_cap, uint32_t min_piece, uint32_t max_piece,
+                             uint32_t grid, hipStream_t st) {
+  int n = n_tiles_total + 1;
+  hipLaunchKernelGGL(tile_starts_kernel, dim3((n + 255) / 256), dim3(256), 0, st, descs, tile_scene, n_tiles_total,
+                     chunk_pre, cbase, offsets, ctl, tile_start, pairs_cap, min_piece, max_piece, grid);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(tile_pieces_kernel, dim3((n_tiles_total + 255) / 256), dim3(256), 0, st, tile_start,
+                     n_tiles_total, ctl, pieces);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(class_bases_kernel, dim3(1), dim3(64), 0, st, ctl, items_cap);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(write_items_kernel, dim3((n_tiles_total + 255) / 256), dim3(256), 0, st, descs, tile_scene,
+                     tile_start, pieces, n_tiles_total, ctl, items, items_cap, tile_split, split_tiles, split_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_accum(const SceneDesc* descs, const uint32_t* pairs, const WorkItem* items,
+                             const uint32_t* tile_split, unsigned long long* split_fx, BatchCtl* ctl, int grid,
+                             hipStream_t st) {
+  hipLaunchKernelGGL((tile_accum_kernel<kTileBlock>), dim3(grid), dim3(kTileBlock), 0, st, descs, pairs, items,
+                     tile_split, split_fx, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_merge(const SceneDesc* descs, const uint32_t* tile_scene, const uint32_t* split_tiles,
+                             const BatchCtl* ctl, unsigned long long* split_fx, uint32_t split_cap, int grid,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(tile_merge_kernel, dim3(grid), dim3(256), 0, st, descs, tile_scene, split_tiles, ctl,
+                     split_fx, split_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_pixels(int mode, const SceneDesc* descs, const uint64_t* px_pre, int n_scenes, uint64_t n_px,
+                         hipStream_t st) {
+  if (n_px == 0) return hipSuccess;
+  unsigned blocks = (unsigned)((n_px + 255) / 256);
+  if (mode == 0)
+    hipLaunchKernelGGL(pixels_kernel<0>, dim3(blocks), dim3(256), 0, st, descs, px_pre, n_scenes, n_px);
+  else
+    hipLaunchKernelGGL(pixels_kernel<1>, dim3(blocks), dim3(256), 0, st, descs, px_pre, n_scenes, n_px);
   return hipGetLastError();
 }
 

@@ -1,50 +1,21 @@
-// lg_api.hip — the C ABI of include/lg.h: contexts, scenes, buffers and the host side of
-// trace / accumulate / resolve / render.
+// lg_api.hip — the C ABI of include/lg.h: contexts, scenes, buffers, images, and the entry
+// points trace / accumulate / resolve / render / batch render, all of which run through the
+// batched frame pipeline (lg_pipeline.hip).
 //
 // The scene API mirrors Tracer (src/light_garden/tracer.rs:4-17, 48-116). A scene keeps
 // the caller's lg_object / lg_light values on the host (FP64, as the reference does) and
 // compiles them into world-space device records on the first trace after an edit.
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
 
-#include "lg_internal.h"
+#include "lg_host.h"
 
 namespace lg {
-
-// ---- launchers defined in lg_trace.hip / lg_accum.hip ------------------------------------
-template <typename T>
-hipError_t launch_trace_gen(const TraceArgs<T>& args, int grid, hipStream_t stream);
-template <typename T>
-size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts);
-int tile_size();
-hipError_t launch_bin_count(const float4* pos, uint64_t n, const AccumView& v, uint32_t chunk,
-                            uint32_t n_chunks, uint32_t* counts, hipStream_t st);
-hipError_t launch_bin_fill(const float4* pos, uint64_t n, const AccumView& v, uint32_t chunk,
-                           uint32_t n_chunks, const uint32_t* offsets, uint32_t* pairs,
-                           hipStream_t st);
-hipError_t launch_tile_starts(const uint32_t* offsets, uint32_t n_chunks, int n_tiles,
-                              const unsigned long long* total64, uint32_t* tile_start,
-                              hipStream_t st);
-hipError_t launch_sum_u64(const uint32_t* in, uint64_t n, unsigned long long* total64,
-                          hipStream_t st);
-uint64_t scan_scratch_elems(uint64_t n);
-hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* scratch,
-                                 hipStream_t st);
-hipError_t launch_tile_accum(const float4* pos, const float4* col0, const float4* col1,
-                             const uint32_t* pairs, const WorkItem* items, uint32_t n_items,
-                             uint32_t* item_counter, const AccumView& v, float4* accum, int grid,
-                             int fixed_bits, hipStream_t st);
-int tile_block();
-int weight_bits();
-hipError_t launch_resolve(const float4* accum, ushort4* out, uint64_t n, hipStream_t st);
-hipError_t launch_fill_zero(float4* p, uint64_t n, hipStream_t st);
 
 // ---- errors --------------------------------------------------------------------------------
 static thread_local std::string g_error;
@@ -56,133 +27,49 @@ int hip_fail(hipError_t e, const char* what) {
   return e == hipErrorOutOfMemory ? LG_E_OOM : LG_E_HIP;
 }
 
-static int fail(int code, const std::string& msg) {
+int fail(int code, const std::string& msg) {
   g_error = msg;
   return code;
 }
 
-// ---- device buffer ---------------------------------------------------------------------------
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  // Grows to at least `need` bytes (contents are not kept).
-  int ensure(size_t need) {
-    if (need <= bytes && p) return LG_OK;
-    reset();
-    size_t b = need ? need : 16;
-    LG_HIP(hipMalloc(&p, b));
-    bytes = b;
-    return LG_OK;
-  }
-  template <typename U>
-  U* as() const {
-    return reinterpret_cast<U*>(p);
-  }
-};
+// ---- buffers -------------------------------------------------------------------------------
+int DevBuf::ensure(size_t need) {
+  if (need <= bytes && p) return LG_OK;
+  reset();
+  size_t b = need ? need : 16;
+  LG_HIP(hipMalloc(&p, b));
+  bytes = b;
+  return LG_OK;
+}
 
-// ---- per-stage timing ---------------------------------------------------------------------
-struct StageRec {
-  std::string name;
-  hipEvent_t start, stop;
-  uint64_t work;
-};
+int PinnedBuf::ensure(size_t need) {
+  if (need <= bytes && p) return LG_OK;
+  reset();
+  size_t b = need ? need : 16;
+  LG_HIP(hipHostMalloc(&p, b));
+  bytes = b;
+  return LG_OK;
+}
 
-}  // namespace lg
+// ---- profiling -----------------------------------------------------------------------------
+StageScope::StageScope(lg_ctx* c, const char* name, uint64_t work) : ctx(c) {
+  if (!ctx->profiling) return;
+  StageRec r;
+  r.name = name;
+  r.work = work;
+  r.start = ctx->get_event();
+  r.stop = ctx->get_event();
+  if (!r.start || !r.stop) return;
+  (void)hipEventRecord(r.start, ctx->stream);
+  ctx->pending.push_back(r);
+  idx = ctx->pending.size() - 1;
+}
 
-using namespace lg;
+StageScope::~StageScope() {
+  if (idx != (size_t)-1) (void)hipEventRecord(ctx->pending[idx].stop, ctx->stream);
+}
 
-struct lg_ctx {
-  int device = 0;
-  int children = 0;      // live scenes, segment buffers and images
-  bool closing = false;  // lg_ctx_destroy was called while children were alive
-  int n_cu = 256;
-  hipStream_t stream = nullptr;
-  bool profiling = false;
-  std::vector<StageRec> pending;        // events of the current call
-  std::vector<hipEvent_t> event_pool;
-  std::vector<lg_stage_time> last_times;
-  // accumulation scratch
-  DevBuf counts, offsets, scan_scratch, tile_start, pairs, items, item_counter, total64;
-  std::vector<uint32_t> h_tile_start;
-  std::vector<WorkItem> h_items;
-  WorkItem* pinned_items = nullptr;
-  size_t pinned_items_cap = 0;
-
-  hipEvent_t get_event() {
-    if (!event_pool.empty()) {
-      hipEvent_t e = event_pool.back();
-      event_pool.pop_back();
-      return e;
-    }
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-  }
-};
-
-struct lg_scene {
-  lg_ctx* ctx = nullptr;
-  std::vector<lg_object> objects;
-  std::vector<lg_light> lights;
-  bool dirty = true;
-  // compiled scene (host copies kept for the starting refractive index)
-  std::vector<Rec<double>> h_recs64;
-  std::vector<int32_t> h_obj_rec;
-  std::vector<double> h_verts64;
-  int n_recs = 0, n_objs = 0, n_verts = 0;
-  DevBuf recs32, recs64, obj_n32, obj_n64, obj_flags, obj_rec, verts32, verts64, lights_dev;
-  // trace scratch
-  DevBuf q0, q1, counters;
-  uint64_t q_cap = 0;       // rays per queue
-  uint64_t seg_hint = 0;    // high-water segment count
-  uint64_t n_primary = 0;
-};
-
-struct lg_segments {
-  lg_ctx* ctx = nullptr;
-  uint32_t flags = 0;
-  uint64_t count = 0, cap = 0;
-  float max_abs = 0.0f;   // largest |colour channel| of the stored segments (host-side bound)
-  DevBuf pos, col0, col1, key, pos64;
-  bool has_pos64 = false;
-};
-
-struct lg_image {
-  lg_ctx* ctx = nullptr;
-  lg_view view{};
-  DevBuf accum, out16;
-};
-
-namespace lg {
-
-// ---- profiling helpers -------------------------------------------------------------------
-struct StageScope {
-  lg_ctx* ctx;
-  size_t idx = (size_t)-1;
-  StageScope(lg_ctx* c, const char* name, uint64_t work) : ctx(c) {
-    if (!ctx->profiling) return;
-    StageRec r;
-    r.name = name;
-    r.work = work;
-    r.start = ctx->get_event();
-    r.stop = ctx->get_event();
-    if (!r.start || !r.stop) return;
-    (void)hipEventRecord(r.start, ctx->stream);
-    ctx->pending.push_back(r);
-    idx = ctx->pending.size() - 1;
-  }
-  ~StageScope() {
-    if (idx != (size_t)-1) (void)hipEventRecord(ctx->pending[idx].stop, ctx->stream);
-  }
-};
-
-static void begin_call(lg_ctx* ctx) {
+void begin_call(lg_ctx* ctx) {
   for (auto& r : ctx->pending) {
     ctx->event_pool.push_back(r.start);
     ctx->event_pool.push_back(r.stop);
@@ -191,7 +78,7 @@ static void begin_call(lg_ctx* ctx) {
 }
 
 // Resolves event times after the stream has been synchronised.
-static void end_call(lg_ctx* ctx) {
+void end_call(lg_ctx* ctx) {
   if (!ctx->profiling) return;
   std::vector<lg_stage_time> out;
   for (auto& r : ctx->pending) {
@@ -389,7 +276,7 @@ static int upload(DevBuf& b, const U* data, size_t n, hipStream_t st) {
   return LG_OK;
 }
 
-static int compile_scene(lg_scene* sc) {
+int compile_scene(lg_scene* sc) {
   if (!sc->dirty) return LG_OK;
   hipStream_t st = sc->ctx->stream;
   std::vector<Rec<double>> recs;
@@ -418,11 +305,10 @@ static int compile_scene(lg_scene* sc) {
     recs32[i].aux = recs[i].aux;
     for (int k = 0; k < 12; k++) recs32[i].p[k] = (float)recs[i].p[k];
   }
-  std::vector<float> n32(n64.begin(), n64.end()), verts32(verts.begin(), verts.end());
+  std::vector<float> verts32(verts.begin(), verts.end());
   int rc;
   if ((rc = upload(sc->recs32, recs32.data(), recs32.size(), st))) return rc;
   if ((rc = upload(sc->recs64, recs.data(), recs.size(), st))) return rc;
-  if ((rc = upload(sc->obj_n32, n32.data(), n32.size(), st))) return rc;
   if ((rc = upload(sc->obj_n64, n64.data(), n64.size(), st))) return rc;
   if ((rc = upload(sc->obj_flags, flags.data(), flags.size(), st))) return rc;
   if ((rc = upload(sc->obj_rec, obj_rec.data(), obj_rec.size(), st))) return rc;
@@ -473,6 +359,10 @@ static int compile_scene(lg_scene* sc) {
   if (base >= (1ull << (64 - kPathBits))) return fail(LG_E_UNSUPPORTED, "scene: too many rays");
   if ((rc = upload(sc->lights_dev, dl.data(), dl.size(), st))) return rc;
   sc->n_primary = base;
+  float lmax = 0.0f;
+  for (const lg_light& L : sc->lights)
+    for (int c = 0; c < 4; c++) lmax = std::max(lmax, std::fabs(L.color[c]));
+  sc->light_max = lmax;
   sc->h_recs64 = recs;
   sc->h_obj_rec = obj_rec;
   sc->h_verts64 = verts;
@@ -483,8 +373,7 @@ static int compile_scene(lg_scene* sc) {
   return LG_OK;
 }
 
-// ---- trace -----------------------------------------------------------------------------------
-static int seg_ensure(lg_segments* s, uint64_t cap, bool need64) {
+int seg_ensure(lg_segments* s, uint64_t cap, bool need64) {
   int rc;
   if (cap > s->cap || !s->pos.p) {
     if ((rc = s->pos.ensure(cap * sizeof(float4)))) return rc;
@@ -500,97 +389,8 @@ static int seg_ensure(lg_segments* s, uint64_t cap, bool need64) {
   return LG_OK;
 }
 
-template <typename T>
-static int trace_impl(lg_scene* sc, const lg_params* P, lg_segments* out) {
-  lg_ctx* ctx = sc->ctx;
-  hipStream_t st = ctx->stream;
-  const bool fp64 = sizeof(T) == 8;
-  const uint64_t n_primary = sc->n_primary;
-  out->count = 0;
-  out->has_pos64 = false;
-  if (n_primary == 0 || P->max_bounce == 0) return LG_OK;
 
-  size_t lds = trace_lds_bytes<T>(sc->n_recs, sc->n_objs, sc->n_verts);
-  if (lds > 64 * 1024) return fail(LG_E_UNSUPPORTED, "scene too large for the LDS-resident trace");
-
-  uint64_t q_need = std::max<uint64_t>(sc->q_cap, std::min<uint64_t>(2 * n_primary, 0xffffffffull));
-  uint64_t seg_need = std::max<uint64_t>(sc->seg_hint, n_primary * std::min<uint32_t>(P->max_bounce, 4u) + 1024);
-  for (int attempt = 0; attempt < 4; attempt++) {
-    int rc;
-    if (q_need > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "ray queue exceeds 2^32 entries");
-    if ((rc = sc->q0.ensure(q_need * sizeof(QRay<double>)))) return rc;
-    if ((rc = sc->q1.ensure(q_need * sizeof(QRay<double>)))) return rc;
-    sc->q_cap = q_need;
-    if ((rc = seg_ensure(out, seg_need, fp64))) return rc;
-    size_t cnt_bytes = 8 + 4 * (size_t)(P->max_bounce + 1);
-    if ((rc = sc->counters.ensure(cnt_bytes))) return rc;
-    LG_HIP(hipMemsetAsync(sc->counters.p, 0, cnt_bytes, st));
-    unsigned long long* seg_count = sc->counters.as<unsigned long long>();
-    uint32_t* gen_counts = reinterpret_cast<uint32_t*>(sc->counters.as<char>() + 8);
-
-    TraceArgs<T> a;
-    std::memset(&a, 0, sizeof(a));
-    a.recs = fp64 ? (const Rec<T>*)sc->recs64.p : (const Rec<T>*)sc->recs32.p;
-    a.recs64 = sc->recs64.as<const Rec<double>>();
-    a.obj_n64 = sc->obj_n64.as<const double>();
-    a.obj_flags = sc->obj_flags.as<const uint32_t>();
-    a.obj_rec = sc->obj_rec.as<const int32_t>();
-    a.verts = fp64 ? (const T*)sc->verts64.p : (const T*)sc->verts32.p;
-    a.verts64 = sc->verts64.as<const double>();
-    a.n_recs = sc->n_recs;
-    a.n_objs = sc->n_objs;
-    a.n_verts = sc->n_verts;
-    a.n_lights = (int)sc->lights.size();
-    a.lights = sc->lights_dev.as<const DevLight>();
-    a.n_primary = n_primary;
-    a.q_cap = (uint32_t)sc->q_cap;
-    for (int c = 0; c < 4; c++) a.cutoff[c] = P->cutoff_color[c];
-    for (int c = 0; c < 4; c++) a.canvas[c] = P->canvas[c];
-    a.seg_pos = out->pos.as<float4>();
-    a.seg_col = out->col0.as<float4>();
-    a.seg_key = out->key.as<uint64_t>();
-    a.seg_pos64 = fp64 ? out->pos64.as<double>() : nullptr;
-    a.seg_count = seg_count;
-    a.seg_cap = out->cap;
-    const int persistent = ctx->n_cu * 8;
-    for (uint32_t g = 0; g < P->max_bounce; g++) {
-      a.gen = g;
-      a.last_gen = (g + 1 == P->max_bounce) ? 1u : 0u;
-      a.in_q = (g & 1) ? sc->q0.as<const QRay<double>>() : sc->q1.as<const QRay<double>>();
-      a.out_q = (g & 1) ? sc->q1.as<QRay<double>>() : sc->q0.as<QRay<double>>();
-      a.in_count = g ? gen_counts + (g - 1) : nullptr;
-      a.out_count = gen_counts + g;
-      uint64_t want = g == 0 ? (n_primary + 255) / 256 : (uint64_t)persistent;
-      int grid = (int)std::min<uint64_t>(std::max<uint64_t>(want, 1), (uint64_t)persistent);
-      StageScope sc_t(ctx, fp64 ? "trace_fp64" : "trace", 0);
-      LG_HIP(launch_trace_gen<T>(a, grid, st));
-    }
-    std::vector<uint32_t> hc(P->max_bounce + 1 + 2);
-    LG_HIP(hipMemcpyAsync(hc.data(), sc->counters.p, cnt_bytes, hipMemcpyDeviceToHost, st));
-    LG_HIP(hipStreamSynchronize(st));
-    uint64_t segs = (uint64_t)hc[0] | ((uint64_t)hc[1] << 32);
-    uint64_t qmax = 0;
-    for (uint32_t g = 0; g < P->max_bounce; g++) qmax = std::max<uint64_t>(qmax, hc[2 + g]);
-    bool over = segs > out->cap || qmax > sc->q_cap;
-    if (!over) {
-      out->count = segs;
-      // children only scale RGB by R or 1 - R, so light colours bound every segment colour
-      float mx = 0.0f;
-      for (const lg_light& L : sc->lights)
-        for (int c = 0; c < 4; c++) mx = std::max(mx, std::fabs(L.color[c]));
-      out->max_abs = mx;
-      out->has_pos64 = fp64;
-      sc->seg_hint = std::max(sc->seg_hint, segs);
-      return LG_OK;
-    }
-    seg_need = std::max<uint64_t>(seg_need, segs + segs / 4 + 1024);
-    q_need = std::max<uint64_t>(q_need, qmax + qmax / 4 + 1024);
-  }
-  return fail(LG_E_CAPACITY, "trace: buffers kept overflowing");
-}
-
-// ---- accumulation --------------------------------------------------------------------------
-static AccumView make_view(const lg_view& v) {
+AccumView make_view(const lg_view& v) {
   AccumView a;
   a.left = v.left;
   a.top = v.top;
@@ -604,110 +404,17 @@ static AccumView make_view(const lg_view& v) {
   return a;
 }
 
-static int accumulate_impl(lg_ctx* ctx, const lg_segments* segs, lg_image* img) {
-  hipStream_t st = ctx->stream;
-  uint64_t n = segs->count;
-  if (n == 0) return LG_OK;
-  if (n > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "accumulate: more than 2^32 segments");
-  AccumView v = make_view(img->view);
-  const int n_tiles = v.tiles_x * v.tiles_y;
-  if (n_tiles > 16384) return fail(LG_E_UNSUPPORTED, "accumulate: image above 8192x8192");
-  // about two chunks per CU: few enough that the [tile][chunk] count matrix stays small
-  uint64_t chunk = std::max<uint64_t>(1024, (n + 2 * (uint64_t)ctx->n_cu - 1) / (2 * (uint64_t)ctx->n_cu));
-  while (((n + chunk - 1) / chunk) * (uint64_t)n_tiles > (64ull << 20)) chunk *= 2;
-  uint32_t n_chunks = (uint32_t)((n + chunk - 1) / chunk);
-  uint64_t n_cells = (uint64_t)n_tiles * n_chunks;
-  int rc;
-  if ((rc = ctx->counts.ensure(n_cells * 4))) return rc;
-  if ((rc = ctx->offsets.ensure(n_cells * 4))) return rc;
-  if ((rc = ctx->scan_scratch.ensure(scan_scratch_elems(n_cells) * 4))) return rc;
-  if ((rc = ctx->tile_start.ensure((n_tiles + 1) * 4))) return rc;
-  if ((rc = ctx->total64.ensure(16))) return rc;
-  const float4* pos = segs->pos.as<const float4>();
-  {
-    StageScope s(ctx, "bin_count", n);
-    LG_HIP(launch_bin_count(pos, n, v, (uint32_t)chunk, n_chunks, ctx->counts.as<uint32_t>(), st));
-  }
-  {
-    StageScope s(ctx, "bin_scan", n_cells);
-    LG_HIP(hipMemsetAsync(ctx->total64.p, 0, 8, st));
-    LG_HIP(launch_sum_u64(ctx->counts.as<uint32_t>(), n_cells, ctx->total64.as<unsigned long long>(), st));
-    LG_HIP(launch_scan_exclusive(ctx->counts.as<uint32_t>(), ctx->offsets.as<uint32_t>(), n_cells,
-                                 ctx->scan_scratch.as<uint32_t>(), st));
-    LG_HIP(launch_tile_starts(ctx->offsets.as<uint32_t>(), n_chunks, n_tiles,
-                              ctx->total64.as<unsigned long long>(), ctx->tile_start.as<uint32_t>(), st));
-  }
-  ctx->h_tile_start.resize(n_tiles + 1);
-  unsigned long long total = 0;
-  LG_HIP(hipMemcpyAsync(ctx->h_tile_start.data(), ctx->tile_start.p, (n_tiles + 1) * 4,
-                        hipMemcpyDeviceToHost, st));
-  LG_HIP(hipMemcpyAsync(&total, ctx->total64.p, 8, hipMemcpyDeviceToHost, st));
-  LG_HIP(hipStreamSynchronize(st));
-  if (total == 0) return LG_OK;
-  if (total > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "accumulate: more than 2^32 segment-tile pairs");
-  if ((rc = ctx->pairs.ensure(total * 4))) return rc;
-  {
-    StageScope s(ctx, "bin_fill", total);
-    LG_HIP(launch_bin_fill(pos, n, v, (uint32_t)chunk, n_chunks, ctx->offsets.as<uint32_t>(),
-                           ctx->pairs.as<uint32_t>(), st));
-  }
-  // work items: tiles split into pieces of at most `piece` pairs, largest first
-  const int grid = ctx->n_cu;
-  const uint64_t kMaxPiece = 1u << 20;
-  uint64_t piece = std::min<uint64_t>(kMaxPiece, std::max<uint64_t>(4096, total / ((uint64_t)grid * 2) + 1));
-  // fixed point of the LDS tile: colour channel c -> round(c * 2^G), weights in 2^-20. A
-  // pixel takes at most one deposit per pair, so an item of <= kMaxPiece pairs stays below
-  // kMaxPiece * 2^20 * 2^31 < 2^63 when |c| * 2^G < 2^31.
-  double bound = std::max<double>(segs->max_abs, (double)segs->max_abs * segs->max_abs);
-  int G = 60;
-  if (bound > 0.0 && std::isfinite(bound)) {
-    double g = std::min(62.0 - weight_bits() - std::log2((double)kMaxPiece), 30.0) - std::log2(bound);
-    G = (int)std::floor(g) - 1;
-    G = std::max(-30, std::min(60, G));
-  }
-  ctx->h_items.clear();
-  for (int t = 0; t < n_tiles; t++) {
-    uint32_t b = ctx->h_tile_start[t], e = ctx->h_tile_start[t + 1];
-    if (e <= b) continue;
-    uint32_t cnt = e - b;
-    uint32_t split = cnt > piece ? 1u : 0u;
-    for (uint32_t p = b; p < e; p += (uint32_t)piece) {
-      WorkItem w;
-      w.tile = (uint32_t)t;
-      w.split = split;
-      w.begin = p;
-      w.end = (uint32_t)std::min<uint64_t>(e, (uint64_t)p + piece);
-      ctx->h_items.push_back(w);
-    }
-  }
-  std::stable_sort(ctx->h_items.begin(), ctx->h_items.end(), [](const WorkItem& a, const WorkItem& b) {
-    return (a.end - a.begin) > (b.end - b.begin);
-  });
-  uint32_t n_items = (uint32_t)ctx->h_items.size();
-  if (ctx->pinned_items_cap < n_items) {
-    if (ctx->pinned_items) (void)hipHostFree(ctx->pinned_items);
-    ctx->pinned_items = nullptr;
-    LG_HIP(hipHostMalloc((void**)&ctx->pinned_items, n_items * sizeof(WorkItem)));
-    ctx->pinned_items_cap = n_items;
-  }
-  std::memcpy(ctx->pinned_items, ctx->h_items.data(), n_items * sizeof(WorkItem));
-  if ((rc = ctx->items.ensure(n_items * sizeof(WorkItem)))) return rc;
-  if ((rc = ctx->item_counter.ensure(16))) return rc;
-  LG_HIP(hipMemcpyAsync(ctx->items.p, ctx->pinned_items, n_items * sizeof(WorkItem),
-                        hipMemcpyHostToDevice, st));
-  LG_HIP(hipMemsetAsync(ctx->item_counter.p, 0, 4, st));
-  {
-    StageScope s(ctx, "tile_accum", total);
-    int g = (int)std::min<uint32_t>(n_items, (uint32_t)grid);
-    LG_HIP(launch_tile_accum(pos, segs->col0.as<const float4>(),
-                             (segs->flags & LG_SEG_VERTEX_COLORS) ? segs->col1.as<const float4>() : nullptr,
-                             ctx->pairs.as<const uint32_t>(), ctx->items.as<const WorkItem>(), n_items,
-                             ctx->item_counter.as<uint32_t>(), v, img->accum.as<float4>(), g, G, st));
-  }
+int check_params(const lg_params* p) {
+  if (!p) return fail(LG_E_INVALID, "params is NULL");
+  if (p->max_bounce > kMaxBounce) return fail(LG_E_UNSUPPORTED, "max_bounce above 23");
+  if (!finite_all(p->canvas, 4) || !(p->canvas[1] > p->canvas[0]) || !(p->canvas[3] > p->canvas[2]))
+    return fail(LG_E_INVALID, "canvas must be {left < right, bottom < top}");
   return LG_OK;
 }
 
 }  // namespace lg
+
+using namespace lg;
 
 // =============================================================================================
 // C ABI
@@ -743,10 +450,10 @@ int lg_ctx_create(int device, lg_ctx** out) {
 
 static void ctx_free(lg_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
+  pipeline_destroy(ctx);
   (void)hipStreamSynchronize(ctx->stream);
   begin_call(ctx);
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
-  if (ctx->pinned_items) (void)hipHostFree(ctx->pinned_items);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -977,14 +684,6 @@ int lg_segments_download_pos64(lg_segments* s, double* pos64, uint64_t cap, uint
 }
 
 // ---- trace ---------------------------------------------------------------------------------
-static int check_params(const lg_params* p) {
-  if (!p) return fail(LG_E_INVALID, "params is NULL");
-  if (p->max_bounce > kMaxBounce) return fail(LG_E_UNSUPPORTED, "max_bounce above 23");
-  if (!finite_all(p->canvas, 4) || !(p->canvas[1] > p->canvas[0]) || !(p->canvas[3] > p->canvas[2]))
-    return fail(LG_E_INVALID, "canvas must be {left < right, bottom < top}");
-  return LG_OK;
-}
-
 int lg_trace(lg_scene* scene, const lg_params* params, lg_segments* out) {
   if (!scene || !out) return fail(LG_E_INVALID, "scene or out is NULL");
   int rc = check_params(params);
@@ -992,13 +691,11 @@ int lg_trace(lg_scene* scene, const lg_params* params, lg_segments* out) {
   lg_ctx* ctx = scene->ctx;
   LG_HIP(hipSetDevice(ctx->device));
   begin_call(ctx);
-  if ((rc = compile_scene(scene))) return rc;
-  if (params->flags & LG_TRACE_FP64)
-    rc = trace_impl<double>(scene, params, out);
-  else
-    rc = trace_impl<float>(scene, params, out);
-  if (rc) return rc;
-  LG_HIP(hipStreamSynchronize(ctx->stream));
+  Entry e;
+  e.scene = scene;
+  e.params = *params;
+  e.segs = out;
+  if ((rc = pipeline_run(ctx, {e}, ST_TRACE))) return rc;
   end_call(ctx);
   return LG_OK;
 }
@@ -1079,9 +776,9 @@ int lg_image_destroy(lg_image* img) {
 
 int lg_image_clear(lg_image* img) {
   if (!img) return fail(LG_E_INVALID, "image is NULL");
-  size_t px = (size_t)img->view.width * img->view.height;
-  LG_HIP(launch_fill_zero(img->accum.as<float4>(), px, img->ctx->stream));
-  return LG_OK;
+  Entry e;
+  e.img = img;
+  return pipeline_run(img->ctx, {e}, ST_CLEAR);
 }
 
 int lg_image_device_ptrs(lg_image* img, void** accum, void** rgba16f) {
@@ -1114,45 +811,72 @@ int lg_accumulate(lg_ctx* ctx, const lg_segments* segs, lg_image* img) {
   if (!ctx || !segs || !img) return fail(LG_E_INVALID, "ctx, segments or image is NULL");
   LG_HIP(hipSetDevice(ctx->device));
   begin_call(ctx);
-  int rc = accumulate_impl(ctx, segs, img);
+  if (segs->count == 0) return LG_OK;
+  if (segs->count > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "accumulate: more than 2^32 segments");
+  Entry e;
+  e.segs = const_cast<lg_segments*>(segs);
+  e.img = img;
+  int rc = pipeline_run(ctx, {e}, ST_ACCUM);
   if (rc) return rc;
-  LG_HIP(hipStreamSynchronize(ctx->stream));
   end_call(ctx);
   return LG_OK;
 }
 
 int lg_resolve(lg_ctx* ctx, lg_image* img) {
   if (!ctx || !img) return fail(LG_E_INVALID, "ctx or image is NULL");
-  size_t px = (size_t)img->view.width * img->view.height;
-  LG_HIP(launch_resolve(img->accum.as<const float4>(), img->out16.as<ushort4>(), px, ctx->stream));
-  return LG_OK;
+  LG_HIP(hipSetDevice(ctx->device));
+  Entry e;
+  e.img = img;
+  return pipeline_run(ctx, {e}, ST_RESOLVE);
 }
 
 int lg_render(lg_scene* scene, const lg_params* params, lg_segments* segs, lg_image* img) {
   if (!scene || !segs || !img) return fail(LG_E_INVALID, "scene, segments or image is NULL");
-  int rc = check_params(params);
-  if (rc) return rc;
-  lg_ctx* ctx = scene->ctx;
+  return lg_batch_render(scene->ctx, 1, &scene, params, &segs, &img);
+}
+
+int lg_batch_render(lg_ctx* ctx, uint32_t n, lg_scene* const* scenes, const lg_params* params,
+                    lg_segments* const* segs, lg_image* const* images) {
+  if (!ctx) return fail(LG_E_INVALID, "ctx is NULL");
+  if (n && (!scenes || !params || !segs || !images)) return fail(LG_E_INVALID, "batch arrays are NULL");
   LG_HIP(hipSetDevice(ctx->device));
   begin_call(ctx);
-  size_t px = (size_t)img->view.width * img->view.height;
-  {
-    StageScope s(ctx, "clear", px);
-    LG_HIP(launch_fill_zero(img->accum.as<float4>(), px, ctx->stream));
+  std::vector<Entry> E(n);
+  for (uint32_t i = 0; i < n; i++) {
+    if (!scenes[i] || !segs[i] || !images[i]) return fail(LG_E_INVALID, "batch entry is NULL");
+    int rc = check_params(&params[i]);
+    if (rc) return rc;
+    E[i].scene = scenes[i];
+    E[i].params = params[i];
+    E[i].segs = segs[i];
+    E[i].img = images[i];
   }
-  if ((rc = compile_scene(scene))) return rc;
-  if (params->flags & LG_TRACE_FP64)
-    rc = trace_impl<double>(scene, params, segs);
-  else
-    rc = trace_impl<float>(scene, params, segs);
+  int rc = pipeline_run(ctx, E, ST_CLEAR | ST_TRACE | ST_ACCUM | ST_RESOLVE);
   if (rc) return rc;
-  if ((rc = accumulate_impl(ctx, segs, img))) return rc;
-  {
-    StageScope s(ctx, "resolve", px);
-    LG_HIP(launch_resolve(img->accum.as<const float4>(), img->out16.as<ushort4>(), px, ctx->stream));
-  }
-  LG_HIP(hipStreamSynchronize(ctx->stream));
   end_call(ctx);
+  return LG_OK;
+}
+
+int lg_ctx_set_graphs(lg_ctx* ctx, int enable) {
+  if (!ctx) return fail(LG_E_INVALID, "ctx is NULL");
+  ctx->use_graphs = enable != 0;
+  return LG_OK;
+}
+
+int lg_ctx_get_tuning(const lg_ctx* ctx, lg_tuning* out) {
+  if (!ctx || !out) return fail(LG_E_INVALID, "ctx or out is NULL");
+  *out = ctx->tune;
+  return LG_OK;
+}
+
+int lg_ctx_set_tuning(lg_ctx* ctx, const lg_tuning* t) {
+  if (!ctx || !t) return fail(LG_E_INVALID, "ctx or tuning is NULL");
+  if (t->rays_per_item < 64 || t->rays_per_item > (1u << 20))
+    return fail(LG_E_INVALID, "rays_per_item must be in [64, 2^20]");
+  if (t->min_piece < 64 || t->max_piece < t->min_piece)
+    return fail(LG_E_INVALID, "tile pieces need 64 <= min_piece <= max_piece");
+  if (t->trace_waves < 1 || t->trace_waves > 16) return fail(LG_E_INVALID, "trace_waves must be in [1, 16]");
+  ctx->tune = *t;
   return LG_OK;
 }
 

@@ -1,0 +1,195 @@
+// lg_host.h — host-side objects behind the C ABI handles, shared by lg_api.hip (the ABI and
+// scene compilation) and lg_pipeline.hip (the batched frame pipeline).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lg_internal.h"
+
+namespace lg {
+
+// ---- device buffer ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grows to at least `need` bytes (contents are not kept).
+  int ensure(size_t need);
+  template <typename U>
+  U* as() const {
+    return reinterpret_cast<U*>(p);
+  }
+};
+
+// Pinned host staging buffer.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int ensure(size_t need);
+  template <typename U>
+  U* as() const {
+    return reinterpret_cast<U*>(p);
+  }
+};
+
+struct StageRec {
+  std::string name;
+  hipEvent_t start, stop;
+  uint64_t work;
+};
+
+struct Pipeline;
+struct PipelineDeleter {
+  void operator()(Pipeline* p) const;
+};
+
+int fail(int code, const std::string& msg);
+
+// ---- kernel launchers (lg_trace.hip, lg_accum.hip) ---------------------------------------------
+template <typename T>
+size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts);
+template <typename T>
+hipError_t launch_trace(const SceneDesc* descs, int n_scenes, uint32_t gen, const uint32_t* item_pre,
+                        BatchCtl* ctl, uint32_t rpi, int cap_recs, int cap_objs, int cap_verts, int grid,
+                        hipStream_t stream);
+hipError_t launch_build_trace_items(const SceneDesc* descs, int n_scenes, uint32_t gen, uint32_t rpi,
+                                    uint32_t* item_pre, BatchCtl* ctl, hipStream_t stream);
+int trace_block();
+int tile_size();
+int tile_block();
+int weight_bits();
+uint64_t scan_scratch_elems(uint64_t n);
+hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* scratch,
+                                 hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_build_chunks(const SceneDesc* descs, int n_scenes, uint32_t chunk_min, uint32_t target,
+                               uint32_t* chunk_pre, uint32_t* cbase, BatchCtl* ctl, hipStream_t st);
+hipError_t launch_bin(int pass, const SceneDesc* descs, int n_scenes, const uint32_t* chunk_pre,
+                      const uint32_t* cbase, BatchCtl* ctl, uint32_t* mat, uint32_t* pairs, uint64_t pairs_cap,
+                      int max_tiles, int grid, hipStream_t st);
+hipError_t launch_sum_u64(const uint32_t* in, uint64_t n, BatchCtl* ctl, hipStream_t st);
+hipError_t launch_work_items(const SceneDesc* descs, const uint32_t* tile_scene, int n_tiles_total,
+                             const uint32_t* chunk_pre, const uint32_t* cbase, const uint32_t* offsets,
+                             BatchCtl* ctl, uint32_t* tile_start, uint32_t* pieces,
+                             WorkItem* items, uint32_t items_cap, uint32_t* tile_split, uint32_t* split_tiles,
+                             uint32_t split_cap, uint64_t pairs_cap, uint32_t min_piece, uint32_t max_piece,
+                             uint32_t grid, hipStream_t st);
+hipError_t launch_tile_accum(const SceneDesc* descs, const uint32_t* pairs, const WorkItem* items,
+                             const uint32_t* tile_split, unsigned long long* split_fx, BatchCtl* ctl, int grid,
+                             hipStream_t st);
+hipError_t launch_tile_merge(const SceneDesc* descs, const uint32_t* tile_scene, const uint32_t* split_tiles,
+                             const BatchCtl* ctl, unsigned long long* split_fx, uint32_t split_cap, int grid,
+                             hipStream_t st);
+uint64_t merge_slot_u64();
+hipError_t launch_pixels(int mode, const SceneDesc* descs, const uint64_t* px_pre, int n_scenes, uint64_t n_px,
+                         hipStream_t st);
+
+}  // namespace lg
+
+struct lg_ctx {
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  int children = 0;      // live scenes, segment buffers and images
+  bool closing = false;  // lg_ctx_destroy was called while children were alive
+  bool profiling = false;
+  bool use_graphs = true;
+  lg_tuning tune{256u, 4096u, 8192u, 4u};
+  std::vector<lg::StageRec> pending;  // events of the current call
+  std::vector<hipEvent_t> event_pool;
+  std::vector<lg_stage_time> last_times;
+  std::unique_ptr<lg::Pipeline, lg::PipelineDeleter> pipe;
+
+  hipEvent_t get_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+
+struct lg_scene {
+  lg_ctx* ctx = nullptr;
+  std::vector<lg_object> objects;
+  std::vector<lg_light> lights;
+  bool dirty = true;
+  std::vector<lg::Rec<double>> h_recs64;
+  std::vector<int32_t> h_obj_rec;
+  std::vector<double> h_verts64;
+  int n_recs = 0, n_objs = 0, n_verts = 0;
+  lg::DevBuf recs32, recs64, obj_n64, obj_flags, obj_rec, verts32, verts64, lights_dev;
+  lg::DevBuf q0, q1;          // ray queues (QRay<double>)
+  uint64_t q_cap = 0;         // rays per queue
+  uint64_t seg_hint = 0;      // high-water segment count
+  uint64_t n_primary = 0;
+  float light_max = 0.0f;     // largest |light colour channel|
+};
+
+struct lg_segments {
+  lg_ctx* ctx = nullptr;
+  uint32_t flags = 0;
+  uint64_t count = 0, cap = 0;
+  float max_abs = 0.0f;       // largest |colour channel| of the stored segments (host-side bound)
+  lg::DevBuf pos, col0, col1, key, pos64;
+  bool has_pos64 = false;
+};
+
+struct lg_image {
+  lg_ctx* ctx = nullptr;
+  lg_view view{};
+  lg::DevBuf accum, out16;
+};
+
+namespace lg {
+
+enum Stage : uint32_t { ST_CLEAR = 1u, ST_TRACE = 2u, ST_ACCUM = 4u, ST_RESOLVE = 8u };
+
+struct Entry {
+  lg_scene* scene = nullptr;
+  lg_params params{};
+  lg_segments* segs = nullptr;
+  lg_image* img = nullptr;
+};
+
+int compile_scene(lg_scene* sc);
+int seg_ensure(lg_segments* s, uint64_t cap, bool need64);
+AccumView make_view(const lg_view& v);
+int check_params(const lg_params* p);
+void begin_call(lg_ctx* ctx);
+void end_call(lg_ctx* ctx);
+
+// Runs `stages` for every entry as one batched frame on ctx's stream and waits for it.
+int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& entries, uint32_t stages);
+void pipeline_destroy(lg_ctx* ctx);
+
+struct StageScope {
+  lg_ctx* ctx;
+  size_t idx = (size_t)-1;
+  StageScope(lg_ctx* c, const char* name, uint64_t work);
+  ~StageScope();
+};
+
+}  // namespace lg

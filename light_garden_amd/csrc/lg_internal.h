@@ -134,41 +134,7 @@ struct alignas(16) QRay {
   uint64_t key;
 };
 
-// ---- trace launch arguments -----------------------------------------------------------
-template <typename T>
-struct TraceArgs {
-  const Rec<T>* recs;         // culling records (precision T), loaded into LDS
-  const Rec<double>* recs64;  // FP64 records: re-evaluation of winners and shading
-  const double* obj_n64;      // refractive index per object
-  const uint32_t* obj_flags;
-  const int32_t* obj_rec;     // first record of each object
-  const T* verts;             // polygon vertices (local frame)
-  const double* verts64;
-  int32_t n_recs;
-  int32_t n_objs;
-  int32_t n_verts;
-  int32_t n_lights;
-  const DevLight* lights;
-  uint64_t n_primary;         // generation 0: rays to generate
-  uint32_t gen;
-  uint32_t last_gen;          // 1: children are not kept (tracer.rs:373, 490)
-  const QRay<double>* in_q;
-  const uint32_t* in_count;   // generation >= 1: device count of in_q
-  QRay<double>* out_q;
-  uint32_t* out_count;
-  uint32_t q_cap;
-  float cutoff[4];
-  double canvas[4];           // left right bottom top
-  // segment output
-  float4* seg_pos;
-  float4* seg_col;
-  uint64_t* seg_key;
-  double* seg_pos64;          // optional, FP64 trace only (4 per segment)
-  unsigned long long* seg_count;
-  uint64_t seg_cap;
-};
-
-// ---- accumulation ---------------------------------------------------------------------
+// ---- batched pipeline: one descriptor per scene ------------------------------------------
 struct AccumView {
   double left, top;
   double sx, sy;              // pixels per world unit
@@ -176,11 +142,97 @@ struct AccumView {
   int32_t tiles_x, tiles_y;
 };
 
+enum SceneFlags : uint32_t {
+  SCENE_FIXED_COUNT = 1u      // accumulate `seg_fixed` uploaded segments (no trace)
+};
+
+// Device pointers kept in descriptors carry the global address space in their type: kernels
+// read them from memory rather than from their arguments, and a plain (generic) pointer
+// loaded from memory makes the compiler emit flat accesses, which tie up more registers and
+// wait on the LDS counter too. gen() turns one back into a plain pointer whose global origin
+// the compiler still sees.
+#define LG_G __attribute__((address_space(1)))
+template <typename T>
+__host__ __device__ __forceinline__ LG_G T* gbl(T* p) {
+  return (LG_G T*)p;
+}
+template <typename T>
+__host__ __device__ __forceinline__ T* gen(LG_G T* p) {
+  return (T*)p;
+}
+
+struct SceneDesc {
+  // geometry (device buffers of the lg_scene)
+  const LG_G void* recs;           // culling records, Rec<float> or Rec<double>
+  const LG_G Rec<double>* recs64;  // FP64 records: re-evaluation of winners and shading
+  const LG_G double* obj_n64;
+  const LG_G uint32_t* obj_flags;
+  const LG_G int32_t* obj_rec;
+  const LG_G void* verts;          // polygon vertices in the culling precision
+  const LG_G double* verts64;
+  const LG_G DevLight* lights;
+  int32_t n_recs, n_objs, n_verts, n_lights;
+  uint64_t n_primary;
+  uint32_t max_bounce;
+  uint32_t flags;             // SceneFlags
+  float cutoff[4];
+  double canvas[4];           // left right bottom top
+  // ray queues (lg_scene)
+  LG_G QRay<double>* q0;
+  LG_G QRay<double>* q1;
+  uint64_t q_cap;
+  // segments (lg_segments)
+  LG_G float4* seg_pos;
+  LG_G float4* seg_col;
+  const LG_G float4* seg_col1;
+  LG_G uint64_t* seg_key;
+  LG_G double* seg_pos64;
+  uint64_t seg_cap;
+  uint64_t seg_fixed;         // SCENE_FIXED_COUNT: number of segments present
+  // per-scene counters (pipeline scratch): segments, then children per generation
+  LG_G unsigned long long* seg_count;
+  LG_G uint32_t* gen_count;
+  // accumulation (lg_image)
+  AccumView view;
+  LG_G float4* accum;
+  LG_G ushort4* out16;
+  int32_t tile_base;          // first global tile of this scene
+  float color_max;            // bound on |colour channel| of the scene's segments
+  uint64_t px_base;           // first pixel of this scene in the batch's pixel range
+};
+
+constexpr int kMaxGen = kPathBits;
+
+// Device-side control block of one batched frame.
+struct BatchCtl {
+  uint32_t n_items_gen[kMaxGen];   // trace items per generation
+  uint32_t item_cursor[kMaxGen];
+  uint32_t n_chunks;
+  uint32_t chunk;                  // segments per bin chunk
+  uint32_t n_counts;               // entries of the [scene][tile][chunk] count matrix
+  uint32_t chunk_cursor[2];        // bin_count, bin_fill
+  uint32_t piece;                  // pairs per work item
+  unsigned long long total_pairs;
+  uint32_t n_work;
+  uint32_t work_cursor;
+  uint32_t flags;                  // CTL_* overflow bits
+  uint32_t n_split;                // tiles split over several work items (merge slots used)
+  // tile work items ordered by size class (floor(log2(pairs))), largest class first
+  uint32_t class_count[32];
+  uint32_t class_base[32];
+  uint32_t class_cursor[32];
+};
+
+enum CtlFlags : uint32_t { CTL_PAIRS_OVERFLOW = 1u, CTL_ITEMS_OVERFLOW = 2u, CTL_SPLIT_OVERFLOW = 4u };
+// Any of these makes the frame's tile stage a no-op; the host grows the buffer and reruns.
+constexpr uint32_t kCtlStop = CTL_PAIRS_OVERFLOW | CTL_ITEMS_OVERFLOW | CTL_SPLIT_OVERFLOW;
+
 struct WorkItem {
-  uint32_t tile;
-  uint32_t split;             // 1: other items share the tile -> flush with atomics
+  uint32_t scene;             // bit 31: the tile is split over several items (merge buffer)
+  uint32_t tile;              // tile index inside the scene
   uint32_t begin, end;        // range in the pair list
 };
+constexpr uint32_t kSplitBit = 0x80000000u;
 
 // ---- error helpers (host) -------------------------------------------------------------
 void set_error(const std::string& msg);

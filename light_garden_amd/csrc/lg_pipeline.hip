@@ -1,0 +1,453 @@
+// lg_pipeline.hip — one batched frame: clear, trace, accumulate and resolve any number of
+// scenes in the same launches.
+//
+// Every stage's work list is built on the device from device-side counts (trace items per
+// bounce generation, bin chunks, tile work items), and every kernel is persistent over its
+// list, so a frame needs no host synchronisation until its end. Buffers are sized from
+// capacities; the end-of-frame readback reports any overflow, the capacities grow and the
+// frame runs again (this happens on the first frames of a scene only). Because the launch
+// sequence is fixed for a given batch shape (buffers and sizes), it is captured once in a
+// hipGraph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lg_host.h"
+
+namespace lg {
+
+constexpr size_t kCounterStride = 128;  // per scene: u64 segments, then u32 per generation
+constexpr uint32_t kChunkMin = 1024;    // segments per bin chunk, at least
+constexpr uint32_t kBinWaves = 4;       // bin workgroups per compute unit (bin_target = 2 per workgroup)
+static_assert(8 + 4 * kMaxGen <= kCounterStride, "counter stride");
+// Pairs per tile work item: total / (2 * n_cu) + 1, clamped to [min_piece, max_piece]. A tile
+// is split when it holds more than one piece. The merge buffer holds a slot per split tile:
+// it starts at 2 * n_cu + 64 slots and grows to what frames report (overflow -> rerun).
+
+struct Pipeline {
+  DevBuf descs, counters, ctl, item_pre, chunk_pre, cbase, counts, offsets, scan_scratch;
+  DevBuf tile_scene, tile_start, pieces, pairs, items, px_pre;
+  DevBuf tile_split, split_tiles, split_fx;  // merge slots of tiles split over several items
+  uint64_t pairs_cap = 1u << 20;
+  uint32_t items_cap = 0;
+  uint32_t split_cap = 0, split_min = 0;
+  PinnedBuf stage_in;   // descs | tile_scene | px_pre
+  PinnedBuf stage_out;  // counters | ctl
+  // Captured frames keyed on their shape (sizes, knobs and every buffer address they bake
+  // in), least recently used evicted first.
+  struct Graph {
+    std::vector<uint64_t> shape;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+  };
+  static constexpr size_t kMaxGraphs = 8;
+  std::vector<Graph> graphs;
+  uint64_t tick = 0;
+
+  static void release(Graph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+  }
+  Graph* find(const std::vector<uint64_t>& shape) {
+    for (Graph& g : graphs)
+      if (g.shape == shape) return &g;
+    return nullptr;
+  }
+  // Room for one more graph.
+  void make_room() {
+    while (graphs.size() >= kMaxGraphs) {
+      size_t lru = 0;
+      for (size_t i = 1; i < graphs.size(); i++)
+        if (graphs[i].used < graphs[lru].used) lru = i;
+      release(graphs[lru]);
+      graphs.erase(graphs.begin() + lru);
+    }
+  }
+  ~Pipeline() {
+    for (Graph& g : graphs) release(g);
+  }
+};
+
+void PipelineDeleter::operator()(Pipeline* p) const { delete p; }
+
+void pipeline_destroy(lg_ctx* ctx) { ctx->pipe.reset(); }
+
+struct Layout {
+  int n = 0;
+  bool fp64 = false;
+  uint32_t max_bounce = 0;
+  int cap_recs = 0, cap_objs = 0, cap_verts = 0;
+  int n_tiles = 0, max_tiles = 0;
+  uint64_t n_counts = 0, n_px = 0;  // n_counts: capacity of the per-frame bin count matrix
+  uint32_t bin_target = 0;           // bin chunks per frame
+  std::vector<int> tile_base;
+  std::vector<uint64_t> px_base;
+};
+
+static int enqueue(lg_ctx* ctx, Pipeline& P, const std::vector<Entry>& E, const Layout& L, uint32_t stages) {
+  hipStream_t st = ctx->stream;
+  const int n = L.n;
+  SceneDesc* d_descs = P.descs.as<SceneDesc>();
+  BatchCtl* d_ctl = P.ctl.as<BatchCtl>();
+  const size_t desc_bytes = n * sizeof(SceneDesc);
+  const size_t tile_bytes = (size_t)L.n_tiles * 4;
+  const size_t px_bytes = (size_t)n * 8;
+  char* in = P.stage_in.as<char>();
+  LG_HIP(hipMemcpyAsync(P.descs.p, in, desc_bytes, hipMemcpyHostToDevice, st));
+  if (tile_bytes) LG_HIP(hipMemcpyAsync(P.tile_scene.p, in + desc_bytes, tile_bytes, hipMemcpyHostToDevice, st));
+  LG_HIP(hipMemcpyAsync(P.px_pre.p, in + desc_bytes + tile_bytes, px_bytes, hipMemcpyHostToDevice, st));
+  LG_HIP(launch_fill_u32(P.counters.as<uint32_t>(), n * kCounterStride / 4, 0u, st));
+  LG_HIP(launch_fill_u32(P.ctl.as<uint32_t>(), sizeof(BatchCtl) / 4, 0u, st));
+  if (stages & ST_CLEAR) {
+    StageScope s(ctx, "clear", L.n_px);
+    LG_HIP(launch_pixels(0, d_descs, P.px_pre.as<uint64_t>(), n, L.n_px, st));
+  }
+  if (stages & ST_TRACE) {
+    StageScope s(ctx, L.fp64 ? "trace_fp64" : "trace", L.max_bounce);
+    const int grid = ctx->n_cu * (int)ctx->tune.trace_waves;
+    for (uint32_t g = 0; g < L.max_bounce; g++) {
+      uint32_t* pre = P.item_pre.as<uint32_t>() + (size_t)g * (n + 1);
+      LG_HIP(launch_build_trace_items(d_descs, n, g, ctx->tune.rays_per_item, pre, d_ctl, st));
+      if (L.fp64)
+        LG_HIP(launch_trace<double>(d_descs, n, g, pre, d_ctl, ctx->tune.rays_per_item, L.cap_recs, L.cap_objs,
+                                    L.cap_verts, grid, st));
+      else
+        LG_HIP(launch_trace<float>(d_descs, n, g, pre, d_ctl, ctx->tune.rays_per_item, L.cap_recs, L.cap_objs,
+                                   L.cap_verts, grid, st));
+    }
+  }
+  if (stages & ST_ACCUM) {
+    const int grid = ctx->n_cu * (int)kBinWaves;
+    {
+      StageScope s(ctx, "bin_count", 0);
+      LG_HIP(launch_build_chunks(d_descs, n, kChunkMin, L.bin_target, P.chunk_pre.as<uint32_t>(),
+                                 P.cbase.as<uint32_t>(), d_ctl, st));
+      LG_HIP(launch_fill_u32(P.counts.as<uint32_t>(), L.n_counts, 0u, st));
+      LG_HIP(launch_bin(0, d_descs, n, P.chunk_pre.as<uint32_t>(), P.cbase.as<uint32_t>(), d_ctl,
+                        P.counts.as<uint32_t>(), nullptr, 0, L.max_tiles, grid, st));
+    }
+    {
+      StageScope s(ctx, "bin_scan", L.n_counts);
+      LG_HIP(launch_sum_u64(P.counts.as<uint32_t>(), L.n_counts, d_ctl, st));
+      LG_HIP(launch_scan_exclusive(P.counts.as<uint32_t>(), P.offsets.as<uint32_t>(), L.n_counts,
+                                   P.scan_scratch.as<uint32_t>(), st));
+      LG_HIP(launch_work_items(d_descs, P.tile_scene.as<uint32_t>(), L.n_tiles, P.chunk_pre.as<uint32_t>(),
+                               P.cbase.as<uint32_t>(), P.offsets.as<uint32_t>(), d_ctl,
+                               P.tile_start.as<uint32_t>(), P.pieces.as<uint32_t>(), P.items.as<WorkItem>(),
+                               P.items_cap, P.tile_split.as<uint32_t>(), P.split_tiles.as<uint32_t>(), P.split_cap,
+                               P.pairs_cap, ctx->tune.min_piece, ctx->tune.max_piece, (uint32_t)ctx->n_cu, st));
+    }
+    {
+      StageScope s(ctx, "bin_fill", 0);
+      LG_HIP(launch_bin(1, d_descs, n, P.chunk_pre.as<uint32_t>(), P.cbase.as<uint32_t>(), d_ctl,
+                        P.offsets.as<uint32_t>(), P.pairs.as<uint32_t>(), P.pairs_cap, L.max_tiles, grid, st));
+    }
+    {
+      StageScope s(ctx, "tile_accum", 0);
+      LG_HIP(launch_tile_accum(d_descs, P.pairs.as<uint32_t>(), P.items.as<WorkItem>(), P.tile_split.as<uint32_t>(),
+                               P.split_fx.as<unsigned long long>(), d_ctl, ctx->n_cu, st));
+    }
+    {
+      StageScope s(ctx, "tile_merge", 0);
+      LG_HIP(launch_tile_merge(d_descs, P.tile_scene.as<uint32_t>(), P.split_tiles.as<uint32_t>(), d_ctl,
+                               P.split_fx.as<unsigned long long>(), P.split_cap, ctx->n_cu, st));
+    }
+  }
+  if (stages & ST_RESOLVE) {
+    StageScope s(ctx, "resolve", L.n_px);
+    LG_HIP(launch_pixels(1, d_descs, P.px_pre.as<uint64_t>(), n, L.n_px, st));
+  }
+  char* out = P.stage_out.as<char>();
+  LG_HIP(hipMemcpyAsync(out, P.counters.p, n * kCounterStride, hipMemcpyDeviceToHost, st));
+  LG_HIP(hipMemcpyAsync(out + n * kCounterStride, P.ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToHost, st));
+  return LG_OK;
+}
+
+int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
+  if (!ctx->pipe) ctx->pipe.reset(new Pipeline());
+  Pipeline& P = *ctx->pipe;
+  const int n = (int)E.size();
+  if (n == 0) return LG_OK;
+  hipStream_t st = ctx->stream;
+  const bool trace = stages & ST_TRACE, accum = stages & ST_ACCUM;
+  const bool pixels = stages & (ST_CLEAR | ST_RESOLVE | ST_ACCUM);
+
+  // ---- validate ----
+  int rc;
+  Layout L;
+  L.n = n;
+  for (int i = 0; i < n; i++) {
+    const Entry& e = E[i];
+    if (trace) {
+      if (!e.scene || !e.segs) return fail(LG_E_INVALID, "batch: trace needs a scene and a segment buffer");
+      if ((rc = check_params(&e.params))) return rc;
+      if (e.scene->ctx != ctx || e.segs->ctx != ctx) return fail(LG_E_INVALID, "batch: objects of another context");
+      bool f = (e.params.flags & LG_TRACE_FP64) != 0;
+      if (i == 0) L.fp64 = f;
+      if (f != L.fp64) return fail(LG_E_INVALID, "batch: mixed FP32/FP64 traces");
+      if ((rc = compile_scene(e.scene))) return rc;
+    }
+    if (accum && !e.segs) return fail(LG_E_INVALID, "batch: accumulation needs a segment buffer");
+    if (pixels && !e.img) return fail(LG_E_INVALID, "batch: an image is needed");
+    if (e.img && e.img->ctx != ctx) return fail(LG_E_INVALID, "batch: image of another context");
+    for (int j = 0; j < i; j++) {
+      if ((e.scene && e.scene == E[j].scene) || (e.segs && e.segs == E[j].segs) || (e.img && e.img == E[j].img))
+        return fail(LG_E_INVALID, "batch: scenes, segment buffers and images must be distinct");
+    }
+  }
+
+  for (int attempt = 0; attempt < 5; attempt++) {
+    // ---- capacities ----
+    L.max_bounce = 0;
+    L.cap_recs = L.cap_objs = L.cap_verts = 1;
+    for (const Entry& e : E) {
+      if (trace) {
+        lg_scene* sc = e.scene;
+        uint64_t qn = std::max<uint64_t>(sc->q_cap, std::min<uint64_t>(2 * sc->n_primary, 0xffffffffull));
+        qn = std::max<uint64_t>(qn, 1);
+        if ((rc = sc->q0.ensure(qn * sizeof(QRay<double>)))) return rc;
+        if ((rc = sc->q1.ensure(qn * sizeof(QRay<double>)))) return rc;
+        sc->q_cap = qn;
+        uint64_t need = std::max<uint64_t>(sc->seg_hint + sc->seg_hint / 8,
+                                           sc->n_primary * std::min<uint32_t>(e.params.max_bounce, 4u) + 1024);
+        need = std::max<uint64_t>(need, e.segs->cap);
+        if ((rc = seg_ensure(e.segs, need, L.fp64))) return rc;
+        L.max_bounce = std::max(L.max_bounce, e.params.max_bounce);
+        L.cap_recs = std::max(L.cap_recs, sc->n_recs);
+        L.cap_objs = std::max(L.cap_objs, sc->n_objs);
+        L.cap_verts = std::max(L.cap_verts, sc->n_verts);
+      }
+    }
+    if (trace) {
+      size_t lds = L.fp64 ? trace_lds_bytes<double>(L.cap_recs, L.cap_objs, L.cap_verts)
+                          : trace_lds_bytes<float>(L.cap_recs, L.cap_objs, L.cap_verts);
+      if (lds > 64 * 1024) return fail(LG_E_UNSUPPORTED, "scene too large for the LDS-resident trace");
+    }
+    // ---- layout ----
+    L.tile_base.assign(n, 0);
+    L.px_base.assign(n, 0);
+    L.n_tiles = 0;
+    L.max_tiles = 1;
+    L.n_counts = 0;
+    L.n_px = 0;
+    L.bin_target = 2u * (uint32_t)ctx->n_cu * kBinWaves;
+    for (int i = 0; i < n; i++) {
+      const Entry& e = E[i];
+      if (e.img) {
+        L.px_base[i] = L.n_px;
+        L.n_px += (uint64_t)e.img->view.width * e.img->view.height;
+      }
+      if (accum) {
+        AccumView v = make_view(e.img->view);
+        int t = v.tiles_x * v.tiles_y;
+        if (t > 16384) return fail(LG_E_UNSUPPORTED, "accumulate: image above 8192x8192");
+        L.tile_base[i] = L.n_tiles;
+        L.n_tiles += t;
+        L.max_tiles = std::max(L.max_tiles, t);
+      }
+    }
+    // chunks hold at least total / bin_target segments, so a scene has at most
+    // n_s * bin_target / total + 1 of them and the batch fewer than bin_target + n; the matrix
+    // has one more entry so that offsets[count] (the end) is always inside it
+    if (accum) L.n_counts = (uint64_t)L.max_tiles * (L.bin_target + (uint64_t)n) + 1;
+    if (L.n_counts > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "batch too large for one frame");
+    // ---- scratch ----
+    // pieces hold at least min_piece pairs, so a frame without pair overflow has fewer than
+    // n_tiles + pairs_cap / min_piece items
+    const uint32_t items_need =
+        (uint32_t)std::min<uint64_t>(0xffffffffull, (uint64_t)L.n_tiles + P.pairs_cap / ctx->tune.min_piece + 1024);
+    P.items_cap = std::max(P.items_cap, items_need);
+    if ((rc = P.descs.ensure(n * sizeof(SceneDesc)))) return rc;
+    if ((rc = P.counters.ensure(n * kCounterStride))) return rc;
+    if ((rc = P.ctl.ensure(sizeof(BatchCtl)))) return rc;
+    if ((rc = P.item_pre.ensure((size_t)kMaxGen * (n + 1) * 4))) return rc;
+    if ((rc = P.chunk_pre.ensure((size_t)(n + 1) * 4))) return rc;
+    if ((rc = P.cbase.ensure((size_t)(n + 1) * 4))) return rc;
+    if ((rc = P.counts.ensure(std::max<uint64_t>(L.n_counts, 1) * 4))) return rc;
+    if ((rc = P.offsets.ensure(std::max<uint64_t>(L.n_counts, 1) * 4))) return rc;
+    uint64_t ss = scan_scratch_elems(L.n_counts);
+    if ((rc = P.scan_scratch.ensure(ss * 4))) return rc;
+    if ((rc = P.tile_scene.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
+    if ((rc = P.tile_start.ensure((size_t)(L.n_tiles + 1) * 4))) return rc;
+    if ((rc = P.pieces.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
+    if ((rc = P.pairs.ensure(P.pairs_cap * 4))) return rc;
+    if ((rc = P.items.ensure((size_t)P.items_cap * sizeof(WorkItem)))) return rc;
+    if ((rc = P.px_pre.ensure((size_t)n * 8))) return rc;
+    {
+      uint64_t sc = std::max<uint64_t>(2ull * ctx->n_cu + 64, P.split_min);
+      sc = std::min<uint64_t>(sc, (uint64_t)std::max(L.n_tiles, 1));
+      P.split_cap = (uint32_t)sc;
+      if ((rc = P.tile_split.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
+      if ((rc = P.split_tiles.ensure((size_t)P.split_cap * 4))) return rc;
+      const size_t fx_bytes = (size_t)P.split_cap * merge_slot_u64() * 8;
+      if (P.split_fx.bytes < fx_bytes) {
+        // slots start zeroed; tile_merge leaves every slot it reads zeroed again
+        if ((rc = P.split_fx.ensure(fx_bytes))) return rc;
+        LG_HIP(launch_fill_u32(P.split_fx.as<uint32_t>(), P.split_fx.bytes / 4, 0u, st));
+        LG_HIP(hipStreamSynchronize(st));
+      }
+    }
+    const size_t desc_bytes = n * sizeof(SceneDesc), tile_bytes = (size_t)L.n_tiles * 4;
+    if ((rc = P.stage_in.ensure(desc_bytes + tile_bytes + (size_t)n * 8))) return rc;
+    if ((rc = P.stage_out.ensure(n * kCounterStride + sizeof(BatchCtl)))) return rc;
+    // ---- descriptors ----
+    char* in = P.stage_in.as<char>();
+    SceneDesc* hd = reinterpret_cast<SceneDesc*>(in);
+    uint32_t* h_tile_scene = reinterpret_cast<uint32_t*>(in + desc_bytes);
+    uint64_t* h_px = reinterpret_cast<uint64_t*>(in + desc_bytes + tile_bytes);
+    for (int i = 0; i < n; i++) {
+      const Entry& e = E[i];
+      SceneDesc& D = hd[i];
+      std::memset(&D, 0, sizeof(D));
+      D.seg_count = gbl(reinterpret_cast<unsigned long long*>(P.counters.as<char>() + i * kCounterStride));
+      D.gen_count = gbl(reinterpret_cast<uint32_t*>(P.counters.as<char>() + i * kCounterStride + 8));
+      float max_abs = 0.0f;
+      if (trace) {
+        lg_scene* sc = e.scene;
+        D.recs = gbl(L.fp64 ? sc->recs64.p : sc->recs32.p);
+        D.recs64 = gbl(sc->recs64.as<const Rec<double>>());
+        D.obj_n64 = gbl(sc->obj_n64.as<const double>());
+        D.obj_flags = gbl(sc->obj_flags.as<const uint32_t>());
+        D.obj_rec = gbl(sc->obj_rec.as<const int32_t>());
+        D.verts = gbl(L.fp64 ? sc->verts64.p : sc->verts32.p);
+        D.verts64 = gbl(sc->verts64.as<const double>());
+        D.lights = gbl(sc->lights_dev.as<const DevLight>());
+        D.n_recs = sc->n_recs;
+        D.n_objs = sc->n_objs;
+        D.n_verts = sc->n_verts;
+        D.n_lights = (int32_t)sc->lights.size();
+        D.n_primary = sc->n_primary;
+        D.max_bounce = e.params.max_bounce;
+        for (int c = 0; c < 4; c++) D.cutoff[c] = e.params.cutoff_color[c];
+        for (int c = 0; c < 4; c++) D.canvas[c] = e.params.canvas[c];
+        D.q0 = gbl(sc->q0.as<QRay<double>>());
+        D.q1 = gbl(sc->q1.as<QRay<double>>());
+        D.q_cap = sc->q_cap;
+        max_abs = sc->light_max;
+      }
+      if (e.segs) {
+        D.seg_pos = gbl(e.segs->pos.as<float4>());
+        D.seg_col = gbl(e.segs->col0.as<float4>());
+        D.seg_col1 = gbl((e.segs->flags & LG_SEG_VERTEX_COLORS) ? e.segs->col1.as<const float4>() : nullptr);
+        D.seg_key = gbl(e.segs->key.as<uint64_t>());
+        D.seg_pos64 = gbl((trace && L.fp64) ? e.segs->pos64.as<double>() : nullptr);
+        D.seg_cap = e.segs->cap;
+        if (!trace) {
+          D.flags |= SCENE_FIXED_COUNT;
+          D.seg_fixed = e.segs->count;
+          max_abs = e.segs->max_abs;
+        }
+      }
+      if (e.img) {
+        D.view = make_view(e.img->view);
+        D.accum = gbl(e.img->accum.as<float4>());
+        D.out16 = gbl(e.img->out16.as<ushort4>());
+        D.px_base = L.px_base[i];
+        h_px[i] = L.px_base[i];
+      } else {
+        h_px[i] = L.n_px;
+      }
+      D.color_max = max_abs;
+      D.tile_base = L.tile_base[i];
+      if (accum) {
+        int t = D.view.tiles_x * D.view.tiles_y;
+        for (int k = 0; k < t; k++) h_tile_scene[L.tile_base[i] + k] = (uint32_t)i;
+      }
+    }
+    // ---- launch (graph replay when the shape is unchanged) ----
+    std::vector<uint64_t> shape = {(uint64_t)n, stages, L.fp64, L.max_bounce, (uint64_t)L.cap_recs,
+                                   (uint64_t)L.cap_objs, (uint64_t)L.cap_verts, (uint64_t)L.n_tiles,
+                                   (uint64_t)L.max_tiles, L.n_counts, L.n_px, L.bin_target, P.pairs_cap, P.items_cap,
+                                   P.split_cap, ctx->tune.rays_per_item, ctx->tune.min_piece, ctx->tune.max_piece,
+                                   ctx->tune.trace_waves};
+    for (DevBuf* b : {&P.descs, &P.counters, &P.ctl, &P.item_pre, &P.chunk_pre, &P.cbase, &P.counts, &P.offsets,
+                      &P.scan_scratch, &P.tile_scene, &P.tile_start, &P.pieces, &P.pairs, &P.items,
+                      &P.px_pre, &P.tile_split, &P.split_tiles, &P.split_fx})
+      shape.push_back((uint64_t)b->p);
+    shape.push_back((uint64_t)P.stage_in.p);
+    shape.push_back((uint64_t)P.stage_out.p);
+    const bool use_graph = ctx->use_graphs && !ctx->profiling;
+    if (use_graph) {
+      Pipeline::Graph* G = P.find(shape);
+      if (!G) {
+        P.make_room();
+        LG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = enqueue(ctx, P, E, L, stages);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+        Pipeline::Graph ng;
+        ng.shape = shape;
+        ng.graph = g;
+        hipError_t ie = hipGraphInstantiate(&ng.exec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          (void)hipGraphDestroy(g);
+          return hip_fail(ie, "hipGraphInstantiate");
+        }
+        P.graphs.push_back(ng);
+        G = &P.graphs.back();
+      }
+      G->used = ++P.tick;
+      LG_HIP(hipGraphLaunch(G->exec, st));
+    } else {
+      if ((rc = enqueue(ctx, P, E, L, stages))) return rc;
+    }
+    LG_HIP(hipStreamSynchronize(st));
+    // ---- overflow check ----
+    const char* out = P.stage_out.as<const char>();
+    const BatchCtl* hc = reinterpret_cast<const BatchCtl*>(out + n * kCounterStride);
+    bool over = false;
+    for (int i = 0; trace && i < n; i++) {
+      const Entry& e = E[i];
+      const uint64_t segs = *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
+      const uint32_t* gc = reinterpret_cast<const uint32_t*>(out + i * kCounterStride + 8);
+      uint64_t qmax = 0;
+      for (uint32_t g = 0; g < e.params.max_bounce; g++) qmax = std::max<uint64_t>(qmax, gc[g]);
+      if (segs > e.segs->cap) {
+        e.scene->seg_hint = std::max<uint64_t>(e.scene->seg_hint, segs + segs / 4 + 1024);
+        over = true;
+      }
+      if (qmax > e.scene->q_cap) {
+        e.scene->q_cap = std::min<uint64_t>(qmax + qmax / 4 + 1024, 0xffffffffull);
+        if (qmax >= 0xffffffffull) return fail(LG_E_UNSUPPORTED, "ray queue exceeds 2^32 entries");
+        over = true;
+      }
+    }
+    if (accum) {
+      if (hc->total_pairs > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "more than 2^32 segment-tile pairs");
+      if (hc->flags & CTL_PAIRS_OVERFLOW) {
+        P.pairs_cap = hc->total_pairs + hc->total_pairs / 4 + 1024;
+        over = true;
+      }
+      if (hc->flags & CTL_ITEMS_OVERFLOW) {
+        P.items_cap *= 2;
+        over = true;
+      }
+      if (hc->flags & CTL_SPLIT_OVERFLOW) {
+        P.split_min = std::max<uint32_t>(hc->n_split + hc->n_split / 4 + 64, P.split_cap);
+        over = true;
+      }
+    }
+    if (!over) {
+      for (int i = 0; trace && i < n; i++) {
+        const Entry& e = E[i];
+        const uint64_t segs = *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
+        e.segs->count = segs;
+        e.segs->has_pos64 = L.fp64;
+        e.segs->max_abs = e.scene->light_max;
+        e.scene->seg_hint = std::max(e.scene->seg_hint, segs);
+      }
+      return LG_OK;
+    }
+  }
+  return fail(LG_E_CAPACITY, "batch: buffers kept overflowing");
+}
+
+}  // namespace lg

@@ -80,7 +80,7 @@ constexpr float kIsland = 1.0e-5f;
 // o + t d spends inside (tin may be negative). Returns false when the line misses.
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__device__ bool chord_circle(const RayT<T>& r, const T* p, const Rec<double>* r64, T& tin,
+__device__ __forceinline__ bool chord_circle(const RayT<T>& r, const T* p, const Rec<double>* r64, T& tin,
                              T& tout) {
   T lx = r.ox - p[0], ly = r.oy - p[1];
   T b = lx * r.dx + ly * r.dy;
@@ -123,7 +123,7 @@ __device__ __forceinline__ void to_local(const RayT<T>& r, const T* p, T& ox, T&
 
 // Rect faces: 0 bottom (0,-1), 1 right (1,0), 2 top (0,1), 3 left (-1,0).
 template <typename T>
-__device__ bool chord_rect(const RayT<T>& r, const T* p, T& tin, T& tout, int& fin, int& fout) {
+__device__ __forceinline__ bool chord_rect(const RayT<T>& r, const T* p, T& tin, T& tout, int& fin, int& fout) {
   T ox, oy, dx, dy;
   to_local(r, p, ox, oy, dx, dy);
   T hw = p[6], hh = p[7];
@@ -156,7 +156,7 @@ __device__ bool chord_rect(const RayT<T>& r, const T* p, T& tin, T& tout, int& f
 }
 
 template <typename T>
-__device__ bool chord_ellipse(const RayT<T>& r, const T* p, const Rec<double>* r64, T& tin,
+__device__ __forceinline__ bool chord_ellipse(const RayT<T>& r, const T* p, const Rec<double>* r64, T& tin,
                               T& tout) {
   T ox, oy, dx, dy;
   to_local(r, p, ox, oy, dx, dy);
@@ -189,7 +189,7 @@ __device__ bool chord_ellipse(const RayT<T>& r, const T* p, const Rec<double>* r
 
 // Convex polygon (counter-clockwise local vertices), Cyrus-Beck clipping of the line.
 template <typename T>
-__device__ bool chord_poly(const RayT<T>& r, const T* p, const T* verts, int first, int nv,
+__device__ __forceinline__ bool chord_poly(const RayT<T>& r, const T* p, const T* verts, int first, int nv,
                            T& tin, T& tout, int& ein, int& eout) {
   T ox, oy, dx, dy;
   to_local(r, p, ox, oy, dx, dy);
@@ -246,7 +246,7 @@ __device__ __forceinline__ T horner3(T A, T B, T C, T D, T s) {
 // bisected (the oracle's rule). Returns false on a miss; sets `ambiguous` (FP32 only)
 // when a sign test at a piece end is too close to zero to trust.
 template <typename T>
-__device__ bool bezier_nearest(const RayT<T>& r, const T* a, T& t_best, T& s_best,
+__device__ __forceinline__ bool bezier_nearest(const RayT<T>& r, const T* a, T& t_best, T& s_best,
                                bool& ambiguous) {
   T nx = -r.dy, ny = r.dx;
   T A = nx * a[0] + ny * a[1];
@@ -423,7 +423,7 @@ __device__ __forceinline__ void consider(Nearest<T>& nb, T t, int rec, int code,
 // Boolean combination of two chords (closed convex operands). Candidate boundary points
 // are offered to `consider` in a-then-b order.
 template <typename T>
-__device__ void logic_candidates(Nearest<T>& nb, int op, bool hitA, T inA, T outA, int ainA,
+__device__ __forceinline__ void logic_candidates(Nearest<T>& nb, int op, bool hitA, T inA, T outA, int ainA,
                                  int aoutA, int recA, bool hitB, T inB, T outB, int ainB,
                                  int aoutB, int recB, int top) {
   const int cA_in = (ainA << 8), cA_out = 1 | (aoutA << 8);
@@ -513,10 +513,11 @@ __device__ __forceinline__ int eval_object(const RayT<T>& r, const Rec<T>* recs,
   return i + 1;
 }
 
-// Walks every record for one ray.
+// Walks every record for one ray. Inlined, so that the compiler sees which records are in
+// LDS and which in global memory.
 template <typename T>
-__device__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>* recs, const Rec<double>* recs64,
-                                  const T* verts, int n_recs) {
+__device__ __forceinline__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>* recs,
+                                                  const Rec<double>* recs64, const T* verts, int n_recs) {
   Nearest<T> nb;
   nearest_init(nb);
   int i = 0;
@@ -526,7 +527,7 @@ __device__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>* recs, const Re
 
 // Outward (closed) or left (open) unit normal of the winning surface at p.
 template <typename T>
-__device__ void hit_normal(const Rec<T>& R, const T* verts, int code, T s, T px, T py, T& nx,
+__device__ __forceinline__ void hit_normal(const Rec<T>& R, const T* verts, int code, T s, T px, T py, T& nx,
                            T& ny) {
   int face = code >> 8;
   switch (R.kind) {
@@ -581,7 +582,7 @@ __device__ void hit_normal(const Rec<T>& R, const T* verts, int code, T s, T px,
 // Ray::refract (tracer.rs:444-450): reflected and refracted directions and the
 // unpolarised Fresnel reflectance; false on total internal reflection.
 template <typename T>
-__device__ bool refract_dir(T dx, T dy, T nx, T ny, T n1, T n2, T& rx, T& ry, T& tx, T& ty,
+__device__ __forceinline__ bool refract_dir(T dx, T dy, T nx, T ny, T n1, T n2, T& rx, T& ry, T& tx, T& ty,
                             T& R) {
   T ci = -(dx * nx + dy * ny);
   if (ci < (T)0) {
@@ -633,7 +634,7 @@ __device__ __forceinline__ bool below_cutoff(const float c[4], const float cut[4
 }
 
 // Light ray i of the scene (light.rs:103-115, 163-174, 225-249), generated in FP64.
-__device__ void gen_ray(const DevLight* lights, int n_lights, uint64_t gi, double& ox, double& oy,
+__device__ __forceinline__ void gen_ray(const DevLight* lights, int n_lights, uint64_t gi, double& ox, double& oy,
                         double& dx, double& dy, float c[4], double& ior) {
   int l = 0;
   while (l + 1 < n_lights && gi >= lights[l + 1].ray_base) l++;
@@ -668,7 +669,7 @@ __device__ void gen_ray(const DevLight* lights, int n_lights, uint64_t gi, doubl
 
 // Canvas exit (tracer.rs:484-486): nearest intersection with the canvas rect.
 template <typename T>
-__device__ bool canvas_exit(const RayT<T>& r, const double* cv, T& px, T& py) {
+__device__ __forceinline__ bool canvas_exit(const RayT<T>& r, const double* cv, T& px, T& py) {
   T t0 = -inf_v<T>(), t1 = inf_v<T>();
   if (r.dx != (T)0) {
     T ta = (cv[0] - r.ox) / r.dx, tb = (cv[1] - r.ox) / r.dx;
@@ -699,7 +700,11 @@ __device__ bool canvas_exit(const RayT<T>& r, const double* cv, T& px, T& py) {
 }
 
 // ---------------------------------------------------------------------------------------
-// One bounce generation.
+// One bounce generation of every scene of a batch.
+//
+// Work items are ranges of `rpi` rays of one scene; item i belongs to the scene s with
+// item_pre[s] <= i < item_pre[s + 1]. A persistent workgroup takes items with an atomic
+// cursor and reloads the scene into LDS only when the scene changes.
 //
 // Ray state (origin, direction, refractive index) is FP64. The O(objects) culling loop runs
 // in T over the LDS records; for T = float the winning object (and a runner-up within the
@@ -709,11 +714,17 @@ __device__ bool canvas_exit(const RayT<T>& r, const double* cv, T& px, T& py) {
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ double tie_tol(double t) { return 1e-5 * (1.0 + t); }
 
-// FP64 answer for a ray whose FP32 culling pass found `nb` (kept out of line: it runs once
-// per ray, the culling loop once per object).
+// Full FP64 scan for the rare ray whose FP32 winner was not confirmed; out of line to keep
+// the main path's registers and code small.
+__device__ __forceinline__ Nearest<double> rescan_fp64(const RayT<double>& rd, const Rec<double>* recs64,
+                                                    const double* verts64, int n_recs) {
+  return nearest_hit<double>(rd, recs64, recs64, verts64, n_recs);
+}
+
+// FP64 answer for a ray whose FP32 culling pass found `nb`.
 __device__ __forceinline__ void refine_fp64(const RayT<double>& rd, const Nearest<float>& nb,
-                                         const Rec<double>* recs64, const double* verts64,
-                                         int n_recs, Nearest<double>& h) {
+                                            const Rec<double>* recs64, const double* verts64,
+                                            int n_recs, Nearest<double>& h) {
   const bool tie = nb.top2 >= 0 && (double)nb.t2 <= (double)nb.t + tie_tol(nb.t);
   if (tie) {
     int a = nb.top < nb.top2 ? nb.top : nb.top2;
@@ -724,209 +735,288 @@ __device__ __forceinline__ void refine_fp64(const RayT<double>& rd, const Neares
     eval_object<double>(rd, recs64, recs64, verts64, nb.top, h);
   }
   const bool unsure = h.top < 0 || (!tie && nb.top2 >= 0 && h.t > (double)nb.t2 - tie_tol(nb.t2));
-  if (unsure) h = nearest_hit<double>(rd, recs64, recs64, verts64, n_recs);
+  if (unsure) h = rescan_fp64(rd, recs64, verts64, n_recs);
+}
+
+__device__ __forceinline__ uint64_t gen_input_count(const SceneDesc& D, uint32_t gen_ix) {
+  if (gen_ix == 0) return D.n_primary;
+  uint64_t c = D.gen_count[gen_ix - 1];
+  return c < D.q_cap ? c : D.q_cap;
+}
+
+// Largest s in [0, n) with pre[s] <= x (pre is non-decreasing, pre[0] = 0).
+__device__ __forceinline__ int find_scene(const uint32_t* pre, int n, uint32_t x) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pre[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
 }
 
 template <typename T, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void trace_gen_kernel(TraceArgs<T> A) {
+__global__ __launch_bounds__(BLOCK) void trace_kernel(const SceneDesc* __restrict__ descs, int n_scenes,
+                                                      uint32_t gen_ix, const uint32_t* __restrict__ item_pre,
+                                                      BatchCtl* ctl, uint32_t rpi, int cap_recs,
+                                                      int cap_objs) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kWaves = BLOCK / 64;
   __shared__ uint32_t s_wseg[kWaves];
   __shared__ uint32_t s_wch[kWaves];
   __shared__ unsigned long long s_seg_base;
   __shared__ uint32_t s_ch_base;
+  __shared__ uint32_t s_item;
 
-  uint64_t count;
-  if (A.gen == 0) {
-    count = A.n_primary;
-  } else {
-    uint32_t c = *A.in_count;
-    count = c < A.q_cap ? c : A.q_cap;
-  }
-  if ((uint64_t)blockIdx.x * BLOCK >= count) return;
-
-  // scene -> LDS
   Rec<T>* s_rec = reinterpret_cast<Rec<T>*>(smem);
-  uint32_t* s_obj_flags = reinterpret_cast<uint32_t*>(s_rec + A.n_recs);
-  int32_t* s_obj_rec = reinterpret_cast<int32_t*>(s_obj_flags + A.n_objs);
+  uint32_t* s_obj_flags = reinterpret_cast<uint32_t*>(s_rec + cap_recs);
+  int32_t* s_obj_rec = reinterpret_cast<int32_t*>(s_obj_flags + cap_objs);
   T* s_verts = reinterpret_cast<T*>(
-      (reinterpret_cast<uintptr_t>(s_obj_rec + A.n_objs) + 15) & ~uintptr_t(15));
-  {
-    const int4* src = reinterpret_cast<const int4*>(A.recs);
-    int4* dst = reinterpret_cast<int4*>(s_rec);
-    int n16 = A.n_recs * (int)(sizeof(Rec<T>) / 16);
-    for (int k = threadIdx.x; k < n16; k += BLOCK) dst[k] = src[k];
-    for (int k = threadIdx.x; k < A.n_objs; k += BLOCK) {
-      s_obj_flags[k] = A.obj_flags[k];
-      s_obj_rec[k] = A.obj_rec[k];
-    }
-    for (int k = threadIdx.x; k < 2 * A.n_verts; k += BLOCK) s_verts[k] = A.verts[k];
-  }
-  __syncthreads();
+      (reinterpret_cast<uintptr_t>(s_obj_rec + cap_objs) + 15) & ~uintptr_t(15));
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  const uint32_t n_items = ctl->n_items_gen[gen_ix];
+  int loaded = -1;
 
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < count; base += stride) {
-    uint64_t idx = base + threadIdx.x;
-    bool active = idx < count;
-    double ox = 0, oy = 0, dx = 1, dy = 0, ior = 1;
-    float c[4] = {0, 0, 0, 0};
-    uint64_t key = 0;
-    if (active) {
-      if (A.gen == 0) {
-        gen_ray(A.lights, A.n_lights, idx, ox, oy, dx, dy, c, ior);
-        key = (idx << kPathBits) | 1ull;
-        if (below_cutoff(c, A.cutoff)) active = false;
-      } else {
-        const QRay<double>& q = A.in_q[idx];
-        ox = q.o[0]; oy = q.o[1]; dx = q.d[0]; dy = q.d[1];
-        c[0] = q.c[0]; c[1] = q.c[1]; c[2] = q.c[2]; c[3] = q.c[3];
-        ior = q.ior;
-        key = q.key;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&ctl->item_cursor[gen_ix], 1u);
+    __syncthreads();
+    // the item (and so the scene) is the same for the whole workgroup: keep it scalar
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= n_items) break;
+    const int sid = __builtin_amdgcn_readfirstlane(find_scene(item_pre, n_scenes, it));
+    const SceneDesc& D = descs[sid];
+    const Rec<double>* recs64 = gen(D.recs64);
+    const double* verts64 = gen(D.verts64);
+    const int n_recs = D.n_recs, n_objs = D.n_objs;
+    if (sid != loaded) {
+      const int4* src = reinterpret_cast<const int4*>(gen(D.recs));
+      int4* dst = reinterpret_cast<int4*>(s_rec);
+      const int n16 = n_recs * (int)(sizeof(Rec<T>) / 16);
+      for (int k = threadIdx.x; k < n16; k += BLOCK) dst[k] = src[k];
+      for (int k = threadIdx.x; k < n_objs; k += BLOCK) {
+        s_obj_flags[k] = gen(D.obj_flags)[k];
+        s_obj_rec[k] = gen(D.obj_rec)[k];
       }
+      const T* vsrc = reinterpret_cast<const T*>(gen(D.verts));
+      for (int k = threadIdx.x; k < 2 * D.n_verts; k += BLOCK) s_verts[k] = vsrc[k];
+      loaded = sid;
     }
+    __syncthreads();
 
-    bool emit = false, keep0 = false, keep1 = false;
-    double px = 0, py = 0;
-    double d0x = 0, d0y = 0, d1x = 0, d1y = 0;
-    float c0[4] = {0, 0, 0, 0}, c1[4] = {0, 0, 0, 0};
-    double n_refr = ior;
-    if (active) {
-      const RayT<double> rd{ox, oy, dx, dy};
-      Nearest<double> h;
-      if constexpr (sizeof(T) == 8) {
-        h = nearest_hit<double>(rd, s_rec, A.recs64, s_verts, A.n_recs);
-      } else {
-        const RayT<float> rf{(float)ox, (float)oy, (float)dx, (float)dy};
-        Nearest<float> nb = nearest_hit<float>(rf, s_rec, A.recs64, s_verts, A.n_recs);
-        nearest_init(h);
-        if (nb.top >= 0) refine_fp64(rd, nb, A.recs64, A.verts64, A.n_recs, h);
-      }
-      if (h.rec >= 0) {
-        const Rec<double>& R = A.recs64[h.rec];
-        if (R.kind == REC_BEZ) {
-          double sp = h.s;
-          px = ((R.p[0] * sp + R.p[2]) * sp + R.p[4]) * sp + R.p[6];
-          py = ((R.p[1] * sp + R.p[3]) * sp + R.p[5]) * sp + R.p[7];
+    const uint64_t count = gen_input_count(D, gen_ix);
+    const uint64_t begin = (uint64_t)(it - item_pre[sid]) * rpi;
+    const uint64_t end = begin + rpi < count ? begin + rpi : count;
+    const bool last_gen = gen_ix + 1 >= D.max_bounce;
+    const QRay<double>* in_q = gen(gen_ix & 1 ? D.q0 : D.q1);
+    QRay<double>* out_q = gen(gen_ix & 1 ? D.q1 : D.q0);
+
+    for (uint64_t base = begin; base < end; base += BLOCK) {
+      const uint64_t idx = base + threadIdx.x;
+      bool active = idx < end;
+      double ox = 0, oy = 0, dx = 1, dy = 0, ior = 1;
+      float c[4] = {0, 0, 0, 0};
+      uint64_t key = 0;
+      if (active) {
+        if (gen_ix == 0) {
+          gen_ray(gen(D.lights), D.n_lights, idx, ox, oy, dx, dy, c, ior);
+          key = (idx << kPathBits) | 1ull;
+          if (below_cutoff(c, D.cutoff)) active = false;
         } else {
-          px = ox + h.t * dx;
-          py = oy + h.t * dy;
+          const QRay<double>& q = in_q[idx];
+          ox = q.o[0]; oy = q.o[1]; dx = q.d[0]; dy = q.d[1];
+          c[0] = q.c[0]; c[1] = q.c[1]; c[2] = q.c[2]; c[3] = q.c[3];
+          ior = q.ior;
+          key = q.key;
         }
-        double nx, ny;
-        hit_normal<double>(R, A.verts64, h.code, h.s, px, py, nx, ny);
-        const int obj = R.obj;
-        const uint32_t fl = s_obj_flags[obj];
-        emit = true;
-        if (fl & OBJ_MATERIAL) {
-          const bool exiting = (fl & OBJ_CLOSED) && (dx * nx + dy * ny > 0.0);
-          double n2 = 1.0;
-          if (exiting) {
-            for (int o = 0; o < A.n_objs; o++) {
-              if (o == obj) continue;
-              if (!(s_obj_flags[o] & OBJ_MATERIAL)) continue;
-              if (contains_object<double>(A.recs64, A.verts64, s_obj_rec[o], px, py)) {
-                n2 = A.obj_n64[o];
-                break;
+      }
+
+      bool emit = false, keep0 = false, keep1 = false;
+      double px = 0, py = 0;
+      double d0x = 0, d0y = 0, d1x = 0, d1y = 0;
+      float c0[4] = {0, 0, 0, 0}, c1[4] = {0, 0, 0, 0};
+      double n_refr = ior;
+      if (active) {
+        const RayT<double> rd{ox, oy, dx, dy};
+        Nearest<double> h;
+        if constexpr (sizeof(T) == 8) {
+          h = nearest_hit<double>(rd, s_rec, recs64, s_verts, n_recs);
+        } else {
+          const RayT<float> rf{(float)ox, (float)oy, (float)dx, (float)dy};
+          Nearest<float> nb = nearest_hit<float>(rf, s_rec, recs64, s_verts, n_recs);
+          nearest_init(h);
+          if (nb.top >= 0) refine_fp64(rd, nb, recs64, verts64, n_recs, h);
+        }
+        if (h.rec >= 0) {
+          const Rec<double>& R = recs64[h.rec];
+          if (R.kind == REC_BEZ) {
+            double sp = h.s;
+            px = ((R.p[0] * sp + R.p[2]) * sp + R.p[4]) * sp + R.p[6];
+            py = ((R.p[1] * sp + R.p[3]) * sp + R.p[5]) * sp + R.p[7];
+          } else {
+            px = ox + h.t * dx;
+            py = oy + h.t * dy;
+          }
+          double nx, ny;
+          hit_normal<double>(R, verts64, h.code, h.s, px, py, nx, ny);
+          const int obj = R.obj;
+          const uint32_t fl = s_obj_flags[obj];
+          emit = true;
+          if (fl & OBJ_MATERIAL) {
+            const bool exiting = (fl & OBJ_CLOSED) && (dx * nx + dy * ny > 0.0);
+            double n2 = 1.0;
+            if (exiting) {
+              for (int o = 0; o < n_objs; o++) {
+                if (o == obj) continue;
+                if (!(s_obj_flags[o] & OBJ_MATERIAL)) continue;
+                if (contains_object<double>(recs64, verts64, s_obj_rec[o], px, py)) {
+                  n2 = gen(D.obj_n64)[o];
+                  break;
+                }
               }
+            } else {
+              n2 = gen(D.obj_n64)[obj];
+            }
+            double Rf;
+            const bool has_refr = refract_dir<double>(dx, dy, nx, ny, ior, n2, d0x, d0y, d1x, d1y, Rf);
+            const float rf = (float)Rf;       // tracer.rs:454
+            const float omr = 1.0f - rf;      // tracer.rs:455
+            c0[0] = c[0] * rf; c0[1] = c[1] * rf; c0[2] = c[2] * rf; c0[3] = c[3];
+            keep0 = !below_cutoff(c0, D.cutoff);
+            if (has_refr) {
+              c1[0] = c[0] * omr; c1[1] = c[1] * omr; c1[2] = c[2] * omr; c1[3] = c[3];
+              keep1 = !below_cutoff(c1, D.cutoff);
+              n_refr = n2;
             }
           } else {
-            n2 = A.obj_n64[obj];
-          }
-          double Rf;
-          const bool has_refr = refract_dir<double>(dx, dy, nx, ny, ior, n2, d0x, d0y, d1x, d1y, Rf);
-          const float rf = (float)Rf;       // tracer.rs:454
-          const float omr = 1.0f - rf;      // tracer.rs:455
-          c0[0] = c[0] * rf; c0[1] = c[1] * rf; c0[2] = c[2] * rf; c0[3] = c[3];
-          keep0 = !below_cutoff(c0, A.cutoff);
-          if (has_refr) {
-            c1[0] = c[0] * omr; c1[1] = c[1] * omr; c1[2] = c[2] * omr; c1[3] = c[3];
-            keep1 = !below_cutoff(c1, A.cutoff);
-            n_refr = n2;
+            // Ray::reflect (tracer.rs:476-480)
+            const double dn = dx * nx + dy * ny;
+            d0x = dx - 2.0 * dn * nx;
+            d0y = dy - 2.0 * dn * ny;
+            normalize_v(d0x, d0y);
+            c0[0] = c[0]; c0[1] = c[1]; c0[2] = c[2]; c0[3] = c[3];
+            keep0 = true;  // same colour as the parent, which passed the cutoff
           }
         } else {
-          // Ray::reflect (tracer.rs:476-480)
-          const double dn = dx * nx + dy * ny;
-          d0x = dx - 2.0 * dn * nx;
-          d0y = dy - 2.0 * dn * ny;
-          normalize_v(d0x, d0y);
-          c0[0] = c[0]; c0[1] = c[1]; c0[2] = c[2]; c0[3] = c[3];
-          keep0 = true;  // same colour as the parent, which passed the cutoff
+          emit = canvas_exit<double>(rd, D.canvas, px, py);
         }
-      } else {
-        emit = canvas_exit<double>(rd, A.canvas, px, py);
+        if (last_gen) {
+          keep0 = false;
+          keep1 = false;
+        }
       }
-      if (A.last_gen) {
-        keep0 = false;
-        keep1 = false;
-      }
-    }
 
-    // ---- workgroup-aggregated append ----
-    const uint64_t m_seg = __ballot(emit);
-    const uint64_t m_c0 = __ballot(keep0);
-    const uint64_t m_c1 = __ballot(keep1);
-    const uint32_t pre_seg = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_seg >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m_seg, 0));
-    const uint32_t pre_c0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_c0 >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m_c0, 0));
-    const uint32_t pre_c1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_c1 >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m_c1, 0));
-    if (lane == 0) {
-      s_wseg[wave] = (uint32_t)__popcll(m_seg);
-      s_wch[wave] = (uint32_t)(__popcll(m_c0) + __popcll(m_c1));
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t ts = 0, tc = 0;
-      for (int w = 0; w < kWaves; w++) {
-        uint32_t a = s_wseg[w], b = s_wch[w];
-        s_wseg[w] = ts;
-        s_wch[w] = tc;
-        ts += a;
-        tc += b;
+      // ---- workgroup-aggregated append ----
+      const uint64_t m_seg = __ballot(emit);
+      const uint64_t m_c0 = __ballot(keep0);
+      const uint64_t m_c1 = __ballot(keep1);
+      const uint32_t pre_seg = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_seg >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m_seg, 0));
+      const uint32_t pre_c0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_c0 >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m_c0, 0));
+      const uint32_t pre_c1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_c1 >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m_c1, 0));
+      if (lane == 0) {
+        s_wseg[wave] = (uint32_t)__popcll(m_seg);
+        s_wch[wave] = (uint32_t)(__popcll(m_c0) + __popcll(m_c1));
       }
-      s_seg_base = ts ? atomicAdd(A.seg_count, (unsigned long long)ts) : 0ull;
-      s_ch_base = tc ? atomicAdd(A.out_count, tc) : 0u;
-    }
-    __syncthreads();
-    if (emit) {
-      const uint64_t slot = s_seg_base + s_wseg[wave] + pre_seg;
-      if (slot < A.seg_cap) {
-        A.seg_pos[slot] = make_float4((float)ox, (float)oy, (float)px, (float)py);
-        A.seg_col[slot] = make_float4(c[0], c[1], c[2], c[3]);
-        A.seg_key[slot] = key;
-        if (A.seg_pos64) {
-          double* p64 = A.seg_pos64 + 4 * slot;
-          p64[0] = ox;
-          p64[1] = oy;
-          p64[2] = px;
-          p64[3] = py;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t ts = 0, tc = 0;
+        for (int w = 0; w < kWaves; w++) {
+          uint32_t a = s_wseg[w], b = s_wch[w];
+          s_wseg[w] = ts;
+          s_wch[w] = tc;
+          ts += a;
+          tc += b;
+        }
+        s_seg_base = ts ? atomicAdd(gen(D.seg_count), (unsigned long long)ts) : 0ull;
+        s_ch_base = tc ? atomicAdd(gen(D.gen_count) + gen_ix, tc) : 0u;
+      }
+      __syncthreads();
+      if (emit) {
+        const uint64_t slot = s_seg_base + s_wseg[wave] + pre_seg;
+        if (slot < D.seg_cap) {
+          gen(D.seg_pos)[slot] = make_float4((float)ox, (float)oy, (float)px, (float)py);
+          gen(D.seg_col)[slot] = make_float4(c[0], c[1], c[2], c[3]);
+          gen(D.seg_key)[slot] = key;
+          if (D.seg_pos64) {
+            double* p64 = gen(D.seg_pos64) + 4 * slot;
+            p64[0] = ox;
+            p64[1] = oy;
+            p64[2] = px;
+            p64[3] = py;
+          }
         }
       }
-    }
-    uint32_t ch_slot = s_ch_base + s_wch[wave] + pre_c0 + pre_c1;
-    if (keep0) {
-      if (ch_slot < A.q_cap) {
-        QRay<double>& q = A.out_q[ch_slot];
-        q.o[0] = px; q.o[1] = py; q.d[0] = d0x; q.d[1] = d0y;
-        q.c[0] = c0[0]; q.c[1] = c0[1]; q.c[2] = c0[2]; q.c[3] = c0[3];
-        q.ior = ior;
-        q.key = child_key(key, 0);
+      uint64_t ch_slot = (uint64_t)s_ch_base + s_wch[wave] + pre_c0 + pre_c1;
+      if (keep0) {
+        if (ch_slot < D.q_cap) {
+          QRay<double>& q = out_q[ch_slot];
+          q.o[0] = px; q.o[1] = py; q.d[0] = d0x; q.d[1] = d0y;
+          q.c[0] = c0[0]; q.c[1] = c0[1]; q.c[2] = c0[2]; q.c[3] = c0[3];
+          q.ior = ior;
+          q.key = child_key(key, 0);
+        }
+        ch_slot++;
       }
-      ch_slot++;
+      if (keep1) {
+        if (ch_slot < D.q_cap) {
+          QRay<double>& q = out_q[ch_slot];
+          q.o[0] = px; q.o[1] = py; q.d[0] = d1x; q.d[1] = d1y;
+          q.c[0] = c1[0]; q.c[1] = c1[1]; q.c[2] = c1[2]; q.c[3] = c1[3];
+          q.ior = n_refr;
+          q.key = child_key(key, 1);
+        }
+      }
+      __syncthreads();
     }
-    if (keep1) {
-      if (ch_slot < A.q_cap) {
-        QRay<double>& q = A.out_q[ch_slot];
-        q.o[0] = px; q.o[1] = py; q.d[0] = d1x; q.d[1] = d1y;
-        q.c[0] = c1[0]; q.c[1] = c1[1]; q.c[2] = c1[2]; q.c[3] = c1[3];
-        q.ior = n_refr;
-        q.key = child_key(key, 1);
+  }
+}
+
+// item_pre[s] = first item of scene s for generation `gen_ix`; item_pre[n] = total. One block.
+__global__ __launch_bounds__(1024) void build_trace_items_kernel(const SceneDesc* __restrict__ descs,
+                                                                 int n_scenes, uint32_t gen_ix, uint32_t rpi,
+                                                                 uint32_t* item_pre, BatchCtl* ctl) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n_scenes; base += 1024) {
+    const int sid = base + threadIdx.x;
+    uint32_t n_it = 0;
+    if (sid < n_scenes) {
+      const SceneDesc& D = descs[sid];
+      if (gen_ix < D.max_bounce) {
+        uint64_t c = gen_input_count(D, gen_ix);
+        n_it = (uint32_t)((c + rpi - 1) / rpi);
       }
     }
+    // block exclusive scan
+    uint32_t x = n_it;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
     __syncthreads();
+    uint32_t wofs = 0;
+    for (int w = 0; w < wave; w++) wofs += s_w[w];
+    const uint32_t carry = s_carry;
+    if (sid < n_scenes) item_pre[sid] = carry + wofs + x - n_it;
+    __syncthreads();
+    if (threadIdx.x == 1023) s_carry = carry + wofs + x;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    item_pre[n_scenes] = s_carry;
+    ctl->n_items_gen[gen_ix] = s_carry;
+    ctl->item_cursor[gen_ix] = 0;
   }
 }
 
@@ -942,16 +1032,31 @@ size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts) {
   return b;
 }
 
+constexpr int kTraceBlock = 256;
+
+int trace_block() { return kTraceBlock; }
+
 template <typename T>
-hipError_t launch_trace_gen(const TraceArgs<T>& args, int grid, hipStream_t stream) {
-  constexpr int kBlock = 256;
-  size_t lds = trace_lds_bytes<T>(args.n_recs, args.n_objs, args.n_verts);
-  hipLaunchKernelGGL((trace_gen_kernel<T, kBlock>), dim3(grid), dim3(kBlock), lds, stream, args);
+hipError_t launch_trace(const SceneDesc* descs, int n_scenes, uint32_t gen, const uint32_t* item_pre,
+                        BatchCtl* ctl, uint32_t rpi, int cap_recs, int cap_objs, int cap_verts, int grid,
+                        hipStream_t stream) {
+  size_t lds = trace_lds_bytes<T>(cap_recs, cap_objs, cap_verts);
+  hipLaunchKernelGGL((trace_kernel<T, kTraceBlock>), dim3(grid), dim3(kTraceBlock), lds, stream, descs,
+                     n_scenes, gen, item_pre, ctl, rpi, cap_recs, cap_objs);
   return hipGetLastError();
 }
 
-template hipError_t launch_trace_gen<float>(const TraceArgs<float>&, int, hipStream_t);
-template hipError_t launch_trace_gen<double>(const TraceArgs<double>&, int, hipStream_t);
+hipError_t launch_build_trace_items(const SceneDesc* descs, int n_scenes, uint32_t gen, uint32_t rpi,
+                                    uint32_t* item_pre, BatchCtl* ctl, hipStream_t stream) {
+  hipLaunchKernelGGL(build_trace_items_kernel, dim3(1), dim3(1024), 0, stream, descs, n_scenes, gen, rpi,
+                     item_pre, ctl);
+  return hipGetLastError();
+}
+
+template hipError_t launch_trace<float>(const SceneDesc*, int, uint32_t, const uint32_t*, BatchCtl*, uint32_t,
+                                        int, int, int, int, hipStream_t);
+template hipError_t launch_trace<double>(const SceneDesc*, int, uint32_t, const uint32_t*, BatchCtl*, uint32_t,
+                                         int, int, int, int, hipStream_t);
 template size_t trace_lds_bytes<float>(int, int, int);
 template size_t trace_lds_bytes<double>(int, int, int);
 

@@ -183,6 +183,25 @@ class Context:
     def set_profiling(self, on: bool):
         abi.call("lg_ctx_set_profiling", self._h, 1 if on else 0)
 
+    def set_graphs(self, on: bool):
+        """Replay frames from hipGraphs while the batch shape is unchanged (default on)."""
+        abi.call("lg_ctx_set_graphs", self._h, 1 if on else 0)
+
+    def tuning(self) -> dict:
+        t = abi.LgTuning()
+        abi.call("lg_ctx_get_tuning", self._h, C.byref(t))
+        return {name: getattr(t, name) for name, _ in abi.LgTuning._fields_}
+
+    def set_tuning(self, **kw):
+        """Changes launch knobs (rays_per_item, min_piece, max_piece, trace_waves); results do not
+        depend on them."""
+        cur = self.tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError("unknown tuning keys %s" % sorted(unknown))
+        cur.update({k: int(v) for k, v in kw.items()})
+        abi.call("lg_ctx_set_tuning", self._h, C.byref(abi.LgTuning(**cur)))
+
     def stage_times(self):
         arr = (abi.LgStageTime * 64)()
         n = C.c_int()
@@ -440,3 +459,18 @@ class Tracer:
             self.close()
         except Exception:
             pass
+
+
+def batch_render(ctx: Context, tracers: Sequence["Tracer"], segs: Sequence[Segments], images: Sequence[Image]):
+    """One frame of every (tracer, segments, image) triple in the same launches
+    (lg_batch_render)."""
+    n = len(tracers)
+    if not (len(segs) == len(images) == n):
+        raise ValueError("tracers, segs and images differ in length")
+    for t in tracers:
+        t.sync()
+    scenes = (C.c_void_p * max(1, n))(*[t._h.value for t in tracers])
+    params = (abi.LgParams * max(1, n))(*[t.params() for t in tracers])
+    sg = (C.c_void_p * max(1, n))(*[s.handle.value for s in segs])
+    im = (C.c_void_p * max(1, n))(*[i.handle.value for i in images])
+    abi.call("lg_batch_render", ctx.handle, n, scenes, params, sg, im)

@@ -43,6 +43,7 @@ def test_struct_layout_matches_c():
     fields = {
         "lg_shape": abi.LgShape, "lg_object": abi.LgObject, "lg_light": abi.LgLight,
         "lg_params": abi.LgParams, "lg_view": abi.LgView, "lg_stage_time": abi.LgStageTime,
+        "lg_tuning": abi.LgTuning,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for cname, cls in fields.items():

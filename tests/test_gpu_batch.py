@@ -1,0 +1,131 @@
+"""Batched frames (lg_batch_render) and hipGraph replay."""
+import numpy as np
+import pytest
+
+import light_garden_amd as lg
+from light_garden_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def tracer_for(ctx, sc, fp64=False):
+    tr = lg.Tracer(ctx, sc.canvas)
+    for o in sc.objects:
+        tr.push_object(o)
+    for l in sc.lights:
+        tr.push_light(l)
+    tr.max_bounce = sc.max_bounce
+    tr.cutoff_color = list(sc.cutoff)
+    tr.fp64 = fp64
+    return tr
+
+
+def small_batch(n=6, ray_scale=0.01):
+    scenes = configs.batched_uneven(n_scenes=n, seed=4, ray_scale=ray_scale)
+    for s in scenes:
+        s.width = s.height = min(s.width, 512)
+    return scenes
+
+
+def test_batch_equals_separate_frames(ctx):
+    """A scene's segments and image are the same bits alone and inside a batch (whose other
+    scenes change the bin chunks, work-item pieces and tile splits)."""
+    scenes = small_batch() + [configs.large_single(num_rays=150000, width=512)]
+    trs = [tracer_for(ctx, s) for s in scenes]
+    segs_b = [lg.Segments(ctx) for _ in scenes]
+    imgs_b = [lg.Image(ctx, s.width, s.height, s.canvas) for s in scenes]
+    lg.batch_render(ctx, trs, segs_b, imgs_b)
+    for s, tr, sb, ib in zip(scenes, trs, segs_b, imgs_b):
+        sa = lg.Segments(ctx)
+        ia = lg.Image(ctx, s.width, s.height, s.canvas)
+        tr.render(sa, ia)
+        pa, ca, _, ka = sa.download()
+        pb, cb, _, kb = sb.download()
+        oa, ob = np.argsort(ka), np.argsort(kb)
+        assert np.array_equal(ka[oa], kb[ob]), s.name
+        assert np.array_equal(pa[oa], pb[ob]), s.name
+        assert np.array_equal(ca[oa], cb[ob]), s.name
+        assert np.array_equal(ia.accum(), ib.accum()), s.name
+        assert np.array_equal(ia.rgba16f(), ib.rgba16f()), s.name
+
+
+def test_batch_matches_oracle(ctx, orc):
+    """Every scene of an uneven batch (1-256 objects, 3-12 bounces, different image sizes)
+    against the FP64 oracle."""
+    scenes = small_batch(n=8, ray_scale=0.004)
+    trs = [tracer_for(ctx, s) for s in scenes]
+    segs = [lg.Segments(ctx) for _ in scenes]
+    imgs = [lg.Image(ctx, s.width, s.height, s.canvas) for s in scenes]
+    lg.batch_render(ctx, trs, segs, imgs)
+    for s, sg, im in zip(scenes, segs, imgs):
+        pos, col, _, key = sg.download()
+        rpos, rcol, rkey = orc.trace_scene(s, threads=8)
+        m = orc.match_segments(key, pos, rkey, rpos)
+        assert m["matched"] >= 0.999 * m["n_b"] and m["diverged"] <= 0.001 * m["n_b"], (s.name, m)
+        ref = orc.accumulate_f64(rpos, rcol, None, s.view(), threads=8)
+        e = orc.rel_l1(im.rgba(), ref)
+        assert e < 1e-3, (s.name, e)
+
+
+def test_graph_replays_match_eager(ctx):
+    """Frames replayed from the captured graph, with other eager work between the replays,
+    equal eager frames bit for bit (the fixed-point tile sums are order independent)."""
+    sc = configs.large_single(num_rays=20000, width=512)
+    tr = tracer_for(ctx, sc)
+    segs = lg.Segments(ctx)
+    img = lg.Image(ctx, sc.width, sc.height, sc.canvas)
+    ctx.set_graphs(False)
+    tr.render(segs, img)
+    ref = img.rgba16f().copy()
+    ref_n = segs.count()
+    ctx.set_graphs(True)
+    other = lg.Image(ctx, 64, 64)
+    o_segs = lg.Segments(ctx, 16)
+    o_segs.upload(np.array([[-0.5, 0.0, 0.5, 0.1]]), np.array([[0.1, 0.2, 0.3, 0.4]], np.float32))
+    for rep in range(4):
+        tr.render(segs, img)
+        assert segs.count() == ref_n
+        assert np.array_equal(img.rgba16f(), ref), rep
+        other.accumulate(o_segs)  # eager work between replays
+        ctx.synchronize()
+    ctx.set_graphs(True)
+
+
+@pytest.mark.parametrize("tune", [dict(rays_per_item=64, trace_waves=1),
+                                  dict(rays_per_item=8192, trace_waves=8),
+                                  dict(min_piece=64, max_piece=64),
+                                  dict(min_piece=1 << 30, max_piece=1 << 30)],
+                         ids=["rpi64", "rpi8192", "split_everything", "never_split"])
+def test_tuning_does_not_change_results(ctx, tune):
+    """Launch knobs change the work lists, never the bits: pieces of 64 pairs split every busy
+    tile over many items merged through the 64-bit slots; 2^30 never splits."""
+    sc = configs.large_single(num_rays=60000, width=512)
+    tr = tracer_for(ctx, sc)
+    default = ctx.tuning()
+
+    def frame():
+        segs = lg.Segments(ctx)
+        img = lg.Image(ctx, sc.width, sc.height, sc.canvas)
+        tr.render(segs, img)
+        p, c, _, k = segs.download()
+        o = np.argsort(k)
+        return k[o], p[o], img.accum(), img.rgba16f()
+
+    ref = frame()
+    ctx.set_tuning(**tune)
+    try:
+        got = frame()
+    finally:
+        ctx.set_tuning(**default)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
+def test_tuning_rejects_bad_values(ctx):
+    from light_garden_amd import abi
+    before = ctx.tuning()
+    for bad in (dict(rays_per_item=1), dict(min_piece=8), dict(min_piece=4096, max_piece=1024),
+                dict(trace_waves=0)):
+        with pytest.raises(abi.LgError):
+            ctx.set_tuning(**bad)
+    assert ctx.tuning() == before
