@@ -1,13 +1,19 @@
-"""Benchmark: ray-segments/s of one frame (trace + accumulate + resolve) of the large single
-scene on one MI355X, with the FP64 oracle's image error, per-kernel roofline, and the
-CPU restatement timed on the host cores beside it.
+"""Benchmark: ray-segments/s of whole frames (trace + bin + tile accumulation + resolve) of the
+large single scene (1M rays, 64 objects, 10 bounces, 2048x2048) on one MI355X, with the
+per-kernel roofline of the dominant kernel and the CPU restatement timed on the host cores
+beside it (that leg also checks this frame's image against the FP64 CPU image).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--rays N]
 Prints one JSON line.
+
+python bench.py --copy-calibration
+Runs only a plain 256 MiB device copy, the reference kernel for scaling rocprofv3's
+FETCH_SIZE / WRITE_SIZE counters to bytes.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -15,11 +21,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "ray-segments/sec (1 GPU) at ≤1e-3 rel L1 image err; p50/p99 scene latency"
-HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector
+# ds_add_u64 lane-operations per second over the whole chip, measured with a dedicated LDS
+# atomic loop on MI355X (no published figure exists); the LDS-atomic roof of tile_accum.
+LDS_U64_ATOMIC_PEAK = 6.0e12
 # Brute-force algorithmic FP32 flops per traced segment of the large scene: 16 circles x 18,
 # 16 rects x 28, 16 lenses x 56, 16 mirrors x 12, plus shading 40 and canvas exit 12.
 FLOPS_PER_SEGMENT_LARGE = 16 * (18 + 28 + 56 + 12) + 52
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "tile_accum_large_single_pmc.md")
 
 
 def cpu_threads():
@@ -40,28 +50,88 @@ def percentile(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
+def measured_traffic():
+    """HBM bytes per tile_accum launch from the committed rocprofv3 --pmc summary, or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def copy_calibration():
+    import torch
+    n = 256 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "launches": 5}))
+
+
+def cpu_leg(scene, img, result, stride):
+    """The oracle (FP64 C restatement) on the host cores: the image check of this frame, then
+    the timed CPU baseline on every stride-th primary ray."""
+    import numpy as np
+    import light_garden_amd as lg
+    from oracle import oracle as orc
+    threads = cpu_threads()
+    # image error of the GPU frame vs the FP64 trace accumulated in FP64
+    rpos, rcol, rkey = orc.trace_scene(scene, threads=threads)
+    ref = orc.accumulate_f64(rpos, rcol, None, scene.view(), threads=threads)
+    result["rel_l1_vs_fp64"] = orc.rel_l1(img.rgba(), ref)
+    result["oracle_segments"] = int(len(rkey))
+    del ref, rpos, rcol, rkey
+    objs = lg.objects_array(scene.objects)
+    lights = lg.lights_array(scene.lights)
+    view = scene.view()
+    times = []
+    nseg_cpu = 0
+    for rep in range(4):  # 1 warm-up + 3 timed
+        acc = np.zeros((scene.height, scene.width, 4), np.float32)
+        t1 = time.perf_counter()
+        pos, col, key = orc.trace(objs, len(scene.objects), lights, len(scene.lights), scene.params(),
+                                  ray_stride=stride, threads=threads)
+        orc.accumulate_f32(pos.astype(np.float32), col, None, view, threads=threads, acc=acc)
+        dt = time.perf_counter() - t1
+        nseg_cpu = len(key)
+        if rep:
+            times.append(dt)
+    med = sorted(times)[len(times) // 2]
+    result["cpu_baseline"] = {
+        "value": nseg_cpu / med, "unit": "ray-segments/s", "cores": threads, "kind": "port",
+        "sample": "every %d-th primary ray of the same scene (%d segments): FP64 trace (chunks of "
+                  "100 rays, %d threads) + FP32 accumulation, median of 3 after 1 warm-up, %.2f s each"
+                  % (stride, nseg_cpu, threads, med),
+    }
+    result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rays", type=int, default=1_000_000)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-check", action="store_true", help="skip the oracle image check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline and image check)")
     ap.add_argument("--cpu-stride", type=int, default=3, help="CPU baseline traces every n-th ray")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (multi-GPU "
-              "sharding is a design note only)" % args.gpus, file=sys.stderr)
+        print("bench.py: every box here has one GPU; this build measures one GPU's share and "
+              "--gpus %d is not supported (sharding is a design note only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
-    import numpy as np
     import light_garden_amd as lg
     from light_garden_amd import abi, configs
 
     if not os.path.exists(abi.LIB_PATH):
-        from light_garden_amd import build as lgb
-        lgb.build()
+        raise SystemExit("bench.py: %s is not built; run __graft_entry__.build()" % abi.LIB_PATH)
 
     scene = configs.large_single(num_rays=args.rays)
     ctx = lg.Context(0)
@@ -79,7 +149,10 @@ def main():
     ctx.synchronize()
     n_seg = segs.count()
 
-    # timed region: K frames replayed from the library's captured hipGraph (its default mode)
+    # timed region: K frames, HIP events around every stage recorded on the library's stream
+    # (the stream its kernels run on)
+    ctx.set_profiling(True)
+    stage_sum = {}
     lat = []
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -87,64 +160,54 @@ def main():
         t1 = time.perf_counter()
         tr.render(segs, img)
         lat.append((time.perf_counter() - t1) * 1e3)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = n_seg * args.steps / elapsed
-
-    # the same K frames launched eagerly with HIP events around every stage, recorded on the
-    # library's stream (the one its kernels run on): per-kernel durations for the roofline
-    ctx.set_profiling(True)
-    stage_sum = {}
-    lat_eager = []
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t1 = time.perf_counter()
-        tr.render(segs, img)
-        lat_eager.append((time.perf_counter() - t1) * 1e3)
         for name, (ms, launches, work) in ctx.stage_times().items():
             s = stage_sum.setdefault(name, [0.0, 0, 0])
             s[0] += ms
             s[1] += launches
             s[2] += work
     ctx.synchronize()
-    elapsed_eager = time.perf_counter() - t0
+    elapsed = time.perf_counter() - t0
     ctx.set_profiling(False)
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n_seg * args.steps / elapsed
 
-    # dominant kernel and its roofline
+    # the same frames replayed from the captured hipGraph (the library's default mode)
+    lat_g = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        tr.render(segs, img)
+        lat_g.append((time.perf_counter() - t1) * 1e3)
+    ctx.synchronize()
+    elapsed_g = time.perf_counter() - t0
+
     stages = {k: {"ms_per_frame": v[0] / args.steps, "launches_per_frame": v[1] / args.steps,
-                  "avg_launch_ms": v[0] / max(1, v[1])} for k, v in stage_sum.items()}
-    dominant = max(stages, key=lambda k: stages[k]["ms_per_frame"])
+                  "avg_launch_ms": v[0] / max(1, v[1]), "work_per_frame": v[2] / args.steps}
+              for k, v in stage_sum.items()}
     px = scene.width * scene.height
-    # algorithmic bytes per launch: tile_accum reads each segment once (16 B end points +
-    # 16 B colour) and reads + writes the FP32 accumulator once (32 B per pixel)
-    algo_bytes = {"tile_accum": 32.0 * n_seg + 32.0 * px, "resolve": 24.0 * px, "clear": 16.0 * px,
-                  "bin_count": 16.0 * n_seg, "bin_fill": 16.0 * n_seg}
-    if dominant.startswith("trace"):
-        # per launch = one bounce generation; report the frame's FLOP rate over all launches
-        flops = FLOPS_PER_SEGMENT_LARGE * n_seg
-        t = stages[dominant]["ms_per_frame"] * 1e-3
-        achieved = flops / t / 1e12
-        roofline = {"bound": "valu", "kernel": dominant, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
-    else:
-        t = stages[dominant]["avg_launch_ms"] * 1e-3
-        b = algo_bytes.get(dominant, 0.0)
-        achieved = b / t / 1e9
-        roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    ta = stages["tile_accum"]
+    samples = ta["work_per_frame"]
+    # algorithmic HBM bytes of one tile_accum launch: every segment read once (16 B end points +
+    # 16 B colour) and the FP32 accumulator read and written once (16 + 16 B per pixel)
+    algo_bytes = 32.0 * n_seg + 32.0 * px
+    t_ta = ta["avg_launch_ms"] * 1e-3
+    achieved = algo_bytes / t_ta / 1e9
+    traffic = measured_traffic()
+    roofline = {"bound": "hbm", "kernel": "tile_accum", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "algorithmic_bytes": algo_bytes, "avg_launch_ms": ta["avg_launch_ms"]}
+    lds_rate = 8.0 * samples / t_ta
     trace_ms = sum(v["ms_per_frame"] for k, v in stages.items() if k.startswith("trace"))
     trace_tflops = FLOPS_PER_SEGMENT_LARGE * n_seg / (trace_ms * 1e-3) / 1e12 if trace_ms else 0.0
-    ta = stages.get("tile_accum")
     secondary = {
+        "tile_accum_lds_atomics": {"achieved": lds_rate, "peak": LDS_U64_ATOMIC_PEAK, "unit": "ds_add_u64/s",
+                                   "frac": lds_rate / LDS_U64_ATOMIC_PEAK, "samples_per_frame": samples,
+                                   "note": "8 LDS u64 atomics per anti-aliased sample (2 pixels x 4 channels)"},
         "trace_valu": {"achieved": trace_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                       "frac": trace_tflops / FP32_PEAK_TFLOPS, "ms_per_frame": trace_ms},
+                       "frac": trace_tflops / FP32_PEAK_TFLOPS, "ms_per_frame": trace_ms,
+                       "note": "brute-force algorithmic flops, %d per segment" % FLOPS_PER_SEGMENT_LARGE},
     }
-    if ta:
-        gbs = algo_bytes["tile_accum"] / (ta["avg_launch_ms"] * 1e-3) / 1e9
-        secondary["tile_accum_hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                       "frac": gbs / HBM_PEAK_GBS, "ms_per_frame": ta["ms_per_frame"]}
 
     result = {
         "metric": METRIC, "value": value, "unit": "ray-segments/s", "n_gpus": 1, "steps": args.steps,
@@ -153,53 +216,17 @@ def main():
         "config": {"workload": "large_single: %d rays (point + 30deg spot + 0.4 directional, seed 2), "
                                "64 objects (16 circle/rect/lens/mirror), 10 bounces, %dx%d RGBA16F"
                                % (scene.n_rays(), scene.width, scene.height),
-                   "segments_per_frame": n_seg, "trace": "fp32 + fp64 steps"},
+                   "segments_per_frame": n_seg, "trace": "fp32 culling + fp64 ray state and shading",
+                   "tuning": ctx.tuning()},
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
-        "eager_profiled": {"value": n_seg * args.steps / elapsed_eager,
-                           "ms_per_step": elapsed_eager * 1e3 / args.steps,
-                           "latency_ms": {"p50": percentile(lat_eager, 50), "p99": percentile(lat_eager, 99)}},
+        "graph_replay": {"value": n_seg * args.steps / elapsed_g, "ms_per_step": elapsed_g * 1e3 / args.steps,
+                         "latency_ms": {"p50": percentile(lat_g, 50), "p99": percentile(lat_g, 99)}},
         "stages": stages,
         "roofline": roofline,
         "roofline_secondary": secondary,
     }
-
-    if not args.no_check or not args.no_cpu:
-        from oracle import oracle as orc
-    if not args.no_check:
-        # image error of this frame vs the FP64 oracle's segments accumulated in FP64
-        threads = cpu_threads()
-        rpos, rcol, rkey = orc.trace_scene(scene, threads=threads)
-        ref = orc.accumulate_f64(rpos, rcol, None, scene.view(), threads=threads)
-        result["rel_l1_vs_fp64"] = orc.rel_l1(img.rgba(), ref)
-        result["oracle_segments"] = int(len(rkey))
-        del ref
-
     if not args.no_cpu:
-        threads = cpu_threads()
-        stride = max(1, args.cpu_stride)
-        objs = lg.objects_array(scene.objects)
-        lights = lg.lights_array(scene.lights)
-        view = scene.view()
-        times = []
-        nseg_cpu = 0
-        for rep in range(4):  # 1 warm-up + 3 timed
-            acc = np.zeros((scene.height, scene.width, 4), np.float32)
-            t1 = time.perf_counter()
-            pos, col, key = orc.trace(objs, len(scene.objects), lights, len(scene.lights), scene.params(),
-                                      ray_stride=stride, threads=threads)
-            orc.accumulate_f32(pos.astype(np.float32), col, None, view, threads=threads, acc=acc)
-            dt = time.perf_counter() - t1
-            nseg_cpu = len(key)
-            if rep:
-                times.append(dt)
-        med = sorted(times)[len(times) // 2]
-        result["cpu_baseline"] = {
-            "value": nseg_cpu / med, "unit": "ray-segments/s", "cores": threads, "kind": "port",
-            "sample": "every %d-th primary ray of the same scene (%d segments): FP64 trace (chunks of "
-                      "100 rays) + FP32 accumulation, median of 3 after 1 warm-up, %.2f s each"
-                      % (stride, nseg_cpu, med),
-        }
-        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+        cpu_leg(scene, img, result, max(1, args.cpu_stride))
     print(json.dumps(result))
 
 

@@ -505,6 +505,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
   constexpr int kWaves = BLOCK / 64;
   if (ctl->flags & kCtlStop) return;  // a buffer overflowed: the host reruns the frame
   const uint32_t n_items = ctl->n_work;
+  unsigned long long wave_samples = 0;  // wave-uniform
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(&ctl->work_cursor, 1u);
     __syncthreads();
@@ -566,6 +567,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
         todo &= todo - 1;
         const int jp = bcast_i(packed, j);
         const int jn = jp & 0xff, jq0 = (jp >> 8) & 0xff, jxm = (jp >> 16) & 1, jvc = (jp >> 17) & 1;
+        wave_samples += (unsigned long long)jn;
         const float jm = bcast_f(m_anchor, j), jsl = bcast_f(slope, j);
         const long long c20_0 = (long long)bcast_i(cq0, j) << kWBits;
         const long long c20_1 = (long long)bcast_i(cq1, j) << kWBits;
@@ -656,6 +658,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
     }
     __syncthreads();
   }
+  if (lane == 0 && wave_samples) atomicAdd(&ctl->samples, wave_samples);
 }
 
 // Converts the merge slots of split tiles into the FP32 accumulator (the same expression as

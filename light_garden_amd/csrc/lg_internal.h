@@ -217,6 +217,7 @@ struct BatchCtl {
   uint32_t work_cursor;
   uint32_t flags;                  // CTL_* overflow bits
   uint32_t n_split;                // tiles split over several work items (merge slots used)
+  unsigned long long samples;      // anti-aliased samples walked by tile_accum (2 pixels x 4 channels each)
   // tile work items ordered by size class (floor(log2(pairs))), largest class first
   uint32_t class_count[32];
   uint32_t class_base[32];

@@ -200,6 +200,7 @@ int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
   }
 
   for (int attempt = 0; attempt < 5; attempt++) {
+    const size_t first_stage = ctx->pending.size();
     // ---- capacities ----
     L.max_bounce = 0;
     L.cap_recs = L.cap_objs = L.cap_verts = 1;
@@ -403,6 +404,16 @@ int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
     // ---- overflow check ----
     const char* out = P.stage_out.as<const char>();
     const BatchCtl* hc = reinterpret_cast<const BatchCtl*>(out + n * kCounterStride);
+    if (ctx->profiling) {  // device-side work counts of this attempt's stages
+      uint64_t segs_total = 0;
+      for (int i = 0; trace && i < n; i++) segs_total += *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
+      for (size_t k = first_stage; k < ctx->pending.size(); k++) {
+        StageRec& r = ctx->pending[k];
+        if (r.name.compare(0, 5, "trace") == 0) r.work = segs_total;
+        if (r.name == "bin_count" || r.name == "bin_fill") r.work = hc->total_pairs;
+        if (r.name == "tile_accum") r.work = hc->samples;
+      }
+    }
     bool over = false;
     for (int i = 0; trace && i < n; i++) {
       const Entry& e = E[i];
