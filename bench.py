@@ -109,6 +109,138 @@ def cpu_leg(scene, img, result, stride):
     result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
 
 
+def stage_table(stage_sum, steps):
+    return {k: {"ms_per_frame": v[0] / steps, "launches_per_frame": v[1] / steps,
+                "avg_launch_ms": v[0] / max(1, v[1]), "work_per_frame": v[2] / steps}
+            for k, v in stage_sum.items()}
+
+
+def add_stage_times(ctx, stage_sum):
+    for name, (ms, launches, work) in ctx.stage_times().items():
+        s = stage_sum.setdefault(name, [0.0, 0, 0])
+        s[0] += ms
+        s[1] += launches
+        s[2] += work
+
+
+def tile_accum_roofline(stages, n_seg, px):
+    ta = stages["tile_accum"]
+    algo_bytes = 32.0 * n_seg + 32.0 * px
+    t_ta = ta["avg_launch_ms"] * 1e-3
+    achieved = algo_bytes / t_ta / 1e9
+    lds_rate = 8.0 * ta["work_per_frame"] / t_ta
+    return ({"bound": "hbm", "kernel": "tile_accum", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo_bytes,
+             "avg_launch_ms": ta["avg_launch_ms"]},
+            {"achieved": lds_rate, "peak": LDS_U64_ATOMIC_PEAK, "unit": "ds_add_u64/s",
+             "frac": lds_rate / LDS_U64_ATOMIC_PEAK, "samples_per_frame": ta["work_per_frame"]})
+
+
+def bench_batched(args):
+    """The 256 uneven scenes: every scene rendered by its own lg_render call (the plain
+    per-scene path), then all of them as one lg_batch_render frame; images compared bit for
+    bit between the two."""
+    import numpy as np
+    import light_garden_amd as lg
+    from light_garden_amd import autotune, configs
+    scenes = configs.batched_uneven(n_scenes=args.scenes, seed=4)
+    ctx = lg.Context(0)
+    trs, segs, imgs = [], [], []
+    for sc in scenes:
+        tr = lg.Tracer(ctx, sc.canvas)
+        for o in sc.objects:
+            tr.push_object(o)
+        for l in sc.lights:
+            tr.push_light(l)
+        tr.max_bounce = sc.max_bounce
+        tr.cutoff_color = list(sc.cutoff)
+        trs.append(tr)
+        segs.append(lg.Segments(ctx))
+        imgs.append(lg.Image(ctx, sc.width, sc.height, sc.canvas))
+    tuned = autotune.apply(ctx, "batched_uneven") if not args.no_tuning_cache else None
+    tuning_source = "tuning_mi355x.json" if tuned else "library defaults"
+
+    def batch_frame():
+        lg.batch_render(ctx, trs, segs, imgs)
+
+    if args.autotune:
+        best, table = autotune.tune(ctx, batch_frame, repeats=2,
+                                    log=lambda m: print("autotune: " + m, file=sys.stderr, flush=True))
+        tuning_source = "autotuned in this run"
+        out = os.path.join(ROOT, "gpurun_out", "tuning_batched_uneven.json")
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        autotune.save("batched_uneven", best, min(r["ms"] for r in table), out,
+                      note="coordinate descent over %d settings" % len(table))
+
+    # ---- plain per-scene path: one lg_render per scene, eager launches ----
+    ctx.set_graphs(False)
+    for tr, sg, im in zip(trs, segs, imgs):  # warm-up: capacities settle
+        tr.render(sg, im)
+    ctx.synchronize()
+    n_seg = sum(s.count() for s in segs)
+    plain_ref = [im.rgba16f().copy() for im in imgs]
+    per_scene = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for tr, sg, im in zip(trs, segs, imgs):
+            t1 = time.perf_counter()
+            tr.render(sg, im)
+            per_scene.append((time.perf_counter() - t1) * 1e3)
+    ctx.synchronize()
+    plain_s = (time.perf_counter() - t0) / args.steps
+    ctx.set_graphs(True)
+
+    # ---- batched: every scene in the same launches ----
+    for _ in range(args.warmup):
+        batch_frame()
+    ctx.synchronize()
+    same = all(np.array_equal(im.rgba16f(), ref) for im, ref in zip(imgs, plain_ref))
+    ctx.set_profiling(True)
+    stage_sum = {}
+    lat = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        batch_frame()
+        lat.append((time.perf_counter() - t1) * 1e3)
+        add_stage_times(ctx, stage_sum)
+    ctx.synchronize()
+    batch_s = (time.perf_counter() - t0) / args.steps
+    ctx.set_profiling(False)
+    lat_g = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        batch_frame()
+        lat_g.append((time.perf_counter() - t1) * 1e3)
+    ctx.synchronize()
+    graph_s = (time.perf_counter() - t0) / args.steps
+    stages = stage_table(stage_sum, args.steps)
+    px = sum(sc.width * sc.height for sc in scenes)
+    roof, lds = tile_accum_roofline(stages, n_seg, px)
+    rays = sum(sc.n_rays() for sc in scenes)
+    result = {
+        "metric": METRIC, "value": n_seg / batch_s, "unit": "ray-segments/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": batch_s * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched_uneven: %d scenes (seed 4), rays log-uniform [1e3, 2e6] (total %d), "
+                               "objects log-uniform [1, 256], max bounce U{3..12}, 512^2-2048^2 RGBA16F (%d px)"
+                               % (len(scenes), rays, px),
+                   "segments_per_frame": n_seg, "tuning": ctx.tuning(), "tuning_source": tuning_source},
+        "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "graph_replay": {"value": n_seg / graph_s, "ms_per_step": graph_s * 1e3,
+                         "latency_ms": {"p50": percentile(lat_g, 50), "p99": percentile(lat_g, 99)}},
+        "plain_per_scene": {"value": n_seg / plain_s, "ms_per_pass": plain_s * 1e3,
+                            "scene_latency_ms": {"p50": percentile(per_scene, 50), "p99": percentile(per_scene, 99)}},
+        "batched_vs_plain": plain_s / min(batch_s, graph_s),
+        "images_identical_to_plain": bool(same),
+        "stages": stages, "roofline": roof, "roofline_secondary": {"tile_accum_lds_atomics": lds},
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +250,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline and image check)")
     ap.add_argument("--cpu-stride", type=int, default=3, help="CPU baseline traces every n-th ray")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--workload", default="large_single", choices=["large_single", "batched"])
+    ap.add_argument("--scenes", type=int, default=256, help="batched: number of scenes")
+    ap.add_argument("--autotune", action="store_true", help="batched: tune launch knobs first")
+    ap.add_argument("--no-tuning-cache", action="store_true", help="batched: ignore tuning_mi355x.json")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: every box here has one GPU; this build measures one GPU's share and "
@@ -125,6 +261,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "batched":
+        bench_batched(args)
         return
 
     import light_garden_amd as lg

@@ -1,0 +1,110 @@
+"""Launch autotuner for the frame pipeline.
+
+The GPU path has four launch knobs (lg_tuning in include/lg.h): rays per trace work item,
+trace workgroups per compute unit, and the smallest / largest tile work item in
+segment-tile pairs. None of them changes results (images are the same bits for every
+setting), only speed, and the best values depend on the workload: a single large scene
+wants small trace items for load balance, a batch of many small scenes wants larger ones.
+
+tune() runs coordinate descent over the knobs on real frames of the caller's workload and
+returns the fastest setting; results are kept in a JSON cache keyed by workload name
+(tuning_mi355x.json next to this file holds the ones measured on MI355X), and apply()
+puts a cached setting on a context.
+"""
+import json
+import os
+import time
+from typing import Callable, Dict, List, Optional, Tuple
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_CACHE = os.path.join(HERE, "tuning_mi355x.json")
+
+SPACE: Dict[str, List[int]] = {
+    "rays_per_item": [128, 256, 512, 1024, 2048],
+    "trace_waves": [2, 4, 8],
+    "max_piece": [4096, 8192, 16384, 32768],
+    "min_piece": [1024, 2048, 4096],
+}
+
+
+def _valid(t: Dict[str, int]) -> bool:
+    return t["min_piece"] <= t["max_piece"]
+
+
+def time_frames(frame: Callable[[], None], sync: Callable[[], None], repeats: int) -> float:
+    """Median wall time (ms) of `repeats` frames after one untimed frame (which also
+    captures the frame's graph and grows buffers)."""
+    frame()
+    sync()
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        frame()
+        sync()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def tune(ctx, frame: Callable[[], None], space: Optional[Dict[str, List[int]]] = None, repeats: int = 3,
+         rounds: int = 2, log: Optional[Callable[[str], None]] = None) -> Tuple[Dict[str, int], List[dict]]:
+    """Coordinate descent: for each knob in turn try every value with the others fixed, keep
+    the fastest; repeat until a round changes nothing. Returns (best tuning, measurements).
+    The context is left on the best tuning."""
+    space = space or SPACE
+    start = ctx.tuning()
+    best = dict(start)
+    table: List[dict] = []
+    cache: Dict[Tuple, float] = {}
+
+    def measure(t):
+        key = tuple(sorted(t.items()))
+        if key not in cache:
+            ctx.set_tuning(**t)
+            ms = time_frames(frame, ctx.synchronize, repeats)
+            cache[key] = ms
+            table.append({"tuning": dict(t), "ms": ms})
+            if log:
+                log("%s -> %.3f ms" % (t, ms))
+        return cache[key]
+
+    best_ms = measure(best)
+    for _ in range(rounds):
+        changed = False
+        for knob, values in space.items():
+            for v in values:
+                cand = dict(best)
+                cand[knob] = v
+                if not _valid(cand):
+                    continue
+                ms = measure(cand)
+                if ms < best_ms * 0.995:  # keep a change only when it is clearly faster
+                    best, best_ms, changed = cand, ms, True
+        if not changed:
+            break
+    ctx.set_tuning(**best)
+    return best, table
+
+
+def load_cache(path: str = DEFAULT_CACHE) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save(workload: str, tuning: Dict[str, int], ms: float, path: str, note: str = "") -> None:
+    data = load_cache(path)
+    data[workload] = {"tuning": tuning, "frame_ms": ms, "note": note}
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def apply(ctx, workload: str, path: str = DEFAULT_CACHE) -> Optional[Dict[str, int]]:
+    """Puts the cached tuning of `workload` on ctx; returns it, or None when none is cached."""
+    entry = load_cache(path).get(workload)
+    if not entry:
+        return None
+    ctx.set_tuning(**entry["tuning"])
+    return entry["tuning"]

@@ -199,7 +199,7 @@ int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
     }
   }
 
-  for (int attempt = 0; attempt < 5; attempt++) {
+  for (int attempt = 0; attempt < 8; attempt++) {
     const size_t first_stage = ctx->pending.size();
     // ---- capacities ----
     L.max_bounce = 0;
