@@ -241,6 +241,114 @@ def bench_batched(args):
     print(json.dumps(result))
 
 
+def bench_serving(args):
+    """Poisson arrivals of render requests over the 256 uneven scenes, 30% of them edits
+    (move or resize one object) of a resident scene, served by the native continuous-
+    batching loop under a fixed HBM budget; loads at 50%, 80% and 95% of the measured
+    batched capacity."""
+    import math
+    import light_garden_amd as lg
+    from light_garden_amd import autotune, configs
+    scenes = configs.batched_uneven(n_scenes=args.scenes, seed=4)
+    ctx = lg.Context(0)
+    autotune.apply(ctx, "batched_uneven")
+    trs, imgs, segs = [], [], []
+    for sc in scenes:
+        tr = lg.Tracer(ctx, sc.canvas)
+        for o in sc.objects:
+            tr.push_object(o)
+        for l in sc.lights:
+            tr.push_light(l)
+        tr.max_bounce = sc.max_bounce
+        tr.cutoff_color = list(sc.cutoff)
+        trs.append(tr)
+        imgs.append(lg.Image(ctx, sc.width, sc.height, sc.canvas))
+        segs.append(lg.Segments(ctx))
+    n = len(scenes)
+    # batched capacity (scenes per second) and the plain per-scene rate on the same scenes
+    for _ in range(2):
+        lg.batch_render(ctx, trs, segs, imgs)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    lg.batch_render(ctx, trs, segs, imgs)
+    t_batch = time.perf_counter() - t0
+    seg_per_scene = [s.count() for s in segs]
+    cap_rps = n / t_batch
+    ctx.set_graphs(False)
+    t0 = time.perf_counter()
+    for tr, sg, im in zip(trs, segs, imgs):
+        tr.render(sg, im)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    ctx.set_graphs(True)
+    print("serving: batched capacity %.1f scenes/s (%.1f ms per %d scenes), plain %.1f scenes/s"
+          % (cap_rps, t_batch * 1e3, n, n / t_plain), file=sys.stderr, flush=True)
+    for s in segs:
+        s.close()
+    budget = args.hbm_budget_gib << 30
+    srv = lg.Server(ctx, hbm_budget=budget)
+    # warm-up: every scene once (its resident state is built)
+    for t in [srv.submit(tr, im) for tr, im in zip(trs, imgs)]:
+        srv.wait(t)
+    rng = configs.SplitMix64(5)
+    loads = {}
+    for load in (0.5, 0.8, 0.95):
+        lam = load * cap_rps
+        reqs = []
+        t = 0.0
+        for _ in range(args.requests):
+            t += -math.log(1.0 - rng.uniform()) / lam
+            k = rng.randint(0, n - 1)
+            edit = None
+            if rng.uniform() < 0.3:
+                ix = rng.randint(0, len(trs[k].objects) - 1)
+                edit = (ix, configs.edit_object(trs[k].objects[ix], rng))
+            reqs.append((t, k, edit))
+        st0 = srv.stats()
+        tickets = []
+        start = time.perf_counter()
+        for at, k, edit in reqs:
+            while True:
+                dt = start + at - time.perf_counter()
+                if dt <= 0:
+                    break
+                if dt > 2e-4:
+                    time.sleep(dt - 1e-4)
+            tickets.append(srv.submit(trs[k], imgs[k], edit=edit))
+        res = [srv.wait(tk) for tk in tickets]
+        st1 = srv.stats()
+        lat = [r["latency_ms"] for r in res]
+        span_s = (max(r["end_us"] for r in res) - min(r["submit_us"] for r in res)) * 1e-6
+        nseg = sum(r["segments"] for r in res)
+        loads["%d%%" % round(load * 100)] = {
+            "offered_rps": lam, "achieved_rps": len(res) / span_s, "segments_per_s": nseg / span_s,
+            "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "max": max(lat)},
+            "mean_batch": sum(r["batch_size"] for r in res) / len(res),
+            "frames": st1["frames"] - st0["frames"], "evictions": st1["evictions"] - st0["evictions"],
+            "requests": len(res), "edits": sum(1 for r in reqs if r[2] is not None)}
+        print("serving: load %.2f -> %s" % (load, json.dumps(loads["%d%%" % round(load * 100)])),
+              file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    top = loads["95%"]
+    plain_sps = sum(seg_per_scene) / t_plain
+    result = {
+        "metric": METRIC, "value": top["segments_per_s"], "unit": "ray-segments/s", "n_gpus": 1,
+        "steps": args.requests, "warmup": n, "ms_per_step": 1e3 / top["achieved_rps"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving: Poisson arrivals over the %d batched_uneven scenes (seed 4), 30%% edits "
+                               "(move/resize one object), HBM budget %d GiB, 2 frames in flight; value at 95%% of "
+                               "batched capacity" % (n, args.hbm_budget_gib),
+                   "batched_capacity_rps": cap_rps, "requests_per_load": args.requests, "tuning": ctx.tuning()},
+        "latency_ms": top["latency_ms"],
+        "loads": loads,
+        "plain_per_scene": {"value": plain_sps, "scenes_per_s": n / t_plain},
+        "served_vs_plain": top["segments_per_s"] / plain_sps,
+        "server_stats": st,
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,7 +358,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline and image check)")
     ap.add_argument("--cpu-stride", type=int, default=3, help="CPU baseline traces every n-th ray")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", default="large_single", choices=["large_single", "batched"])
+    ap.add_argument("--workload", default="large_single", choices=["large_single", "batched", "serving"])
+    ap.add_argument("--requests", type=int, default=1024, help="serving: requests per load level")
+    ap.add_argument("--hbm-budget-gib", type=int, default=64, help="serving: HBM budget")
     ap.add_argument("--scenes", type=int, default=256, help="batched: number of scenes")
     ap.add_argument("--autotune", action="store_true", help="batched: tune launch knobs first")
     ap.add_argument("--no-tuning-cache", action="store_true", help="batched: ignore tuning_mi355x.json")
@@ -264,6 +374,9 @@ def main():
         return
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
 
     import light_garden_amd as lg

@@ -251,6 +251,54 @@ int lg_ctx_set_tuning(lg_ctx* ctx, const lg_tuning* tuning);
 /* Replay frames from a hipGraph while the batch keeps its shape (default on). */
 int lg_ctx_set_graphs(lg_ctx* ctx, int enable);
 
+/* ---- serving (no reference counterpart: the reference renders one scene per event-loop
+ * frame, LightGarden::draw mod.rs:680-696; this is the batch/serve seam lg_server_*) ----
+ * A server owns a worker thread that keeps up to `slots` batched frames in flight, each on a
+ * stream of its own, and forms the next frame from waiting requests whenever a slot frees up
+ * (continuous batching). Requests are admitted in arrival order while the resident working
+ * set fits the HBM budget: every resident scene's ray queues and segment buffer (owned by the
+ * server, kept across requests and edits) plus the images of admitted requests and the
+ * frames' scratch; idle scenes are evicted least-recently-used first. Requests for one scene
+ * run in submission order; an edit replaces one object right before its render. A scene's
+ * image is the same bits whatever else shares its frame.
+ * Scenes and images given to a server are the server's until their requests complete (do
+ * not change or render them meanwhile); destroy the server before them. */
+typedef struct lg_server lg_server;
+typedef struct lg_server_config {
+  uint64_t hbm_budget;  /* bytes; 0 = 64 GiB */
+  uint32_t max_batch;   /* scenes per frame; 0 = 256 */
+  uint32_t slots;       /* frames in flight, 1..4; 0 = 2 */
+} lg_server_config;
+typedef struct lg_request {
+  lg_scene* scene;      /* resident scene, rendered with `params` into `image` (lg_render) */
+  lg_image* image;
+  lg_params params;     /* FP32 trace */
+  int32_t edit_index;   /* >= 0: lg_scene_replace_object(scene, edit_index, &edit) first */
+  int32_t _pad0;
+  lg_object edit;
+} lg_request;
+typedef struct lg_result {
+  int32_t status;       /* LG_OK or the error of the frame that rendered the request */
+  uint32_t batch_size;  /* requests in that frame */
+  uint64_t segments;    /* segments traced for the request */
+  double submit_us;     /* server clock, microseconds: submitted, */
+  double start_us;      /* its frame launched, */
+  double end_us;        /* its frame completed */
+} lg_result;
+typedef struct lg_server_stats {
+  uint64_t submitted, completed, frames, evictions;
+  uint64_t resident_bytes, peak_resident_bytes;
+  double busy_us;       /* time with at least one frame in flight */
+} lg_server_stats;
+int lg_server_create(lg_ctx* ctx, const lg_server_config* cfg, lg_server** out);
+/* Finishes every submitted request, then stops. */
+int lg_server_destroy(lg_server* server);
+int lg_server_submit(lg_server* server, const lg_request* request, uint64_t* ticket);
+/* Waits up to timeout_s (< 0: forever) for a request; LG_E_TIMEOUT if it has not finished.
+ * Each ticket's result is returned once. */
+int lg_server_wait(lg_server* server, uint64_t ticket, double timeout_s, lg_result* out);
+int lg_server_get_stats(lg_server* server, lg_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,10 +10,11 @@ from .geo import CubicBezier, Circle, ConvexPolygon, Ellipse, LineSegment, Logic
 from .tracer import (Context, DirectionalLight, Image, Material, Object, PointLight, Segments, SpotLight,
                      Tracer, batch_render, lights_array, objects_array)
 from .string_mod import ModRemColor, StringMod
+from .server import Server
 from . import configs
 
 __all__ = [
     "abi", "geo", "configs", "CubicBezier", "Circle", "ConvexPolygon", "Ellipse", "LineSegment", "Logic",
     "Rect", "lens", "rotation", "Context", "DirectionalLight", "Image", "Material", "Object", "PointLight",
-    "Segments", "SpotLight", "Tracer", "batch_render", "lights_array", "objects_array", "ModRemColor", "StringMod",
+    "Segments", "SpotLight", "Tracer", "batch_render", "lights_array", "objects_array", "ModRemColor", "StringMod", "Server",
 ]

@@ -92,6 +92,26 @@ class LgTuning(C.Structure):
                 ("trace_waves", C.c_uint32)]
 
 
+class LgServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_uint32), ("slots", C.c_uint32)]
+
+
+class LgRequest(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("image", C.c_void_p), ("params", LgParams), ("edit_index", C.c_int32),
+                ("_pad0", C.c_int32), ("edit", LgObject)]
+
+
+class LgResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("batch_size", C.c_uint32), ("segments", C.c_uint64),
+                ("submit_us", C.c_double), ("start_us", C.c_double), ("end_us", C.c_double)]
+
+
+class LgServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("frames", C.c_uint64),
+                ("evictions", C.c_uint64), ("resident_bytes", C.c_uint64), ("peak_resident_bytes", C.c_uint64),
+                ("busy_us", C.c_double)]
+
+
 assert C.sizeof(LgShape) == 136
 assert C.sizeof(LgObject) == 16 + 7 * 136 + 256
 assert C.sizeof(LgLight) == 72
@@ -145,6 +165,11 @@ FUNCTIONS = {
     "lg_ctx_set_graphs": (C.c_int, [_P, C.c_int]),
     "lg_ctx_get_tuning": (C.c_int, [_P, C.POINTER(LgTuning)]),
     "lg_ctx_set_tuning": (C.c_int, [_P, C.POINTER(LgTuning)]),
+    "lg_server_create": (C.c_int, [_P, C.POINTER(LgServerConfig), _PP]),
+    "lg_server_destroy": (C.c_int, [_P]),
+    "lg_server_submit": (C.c_int, [_P, C.POINTER(LgRequest), C.POINTER(C.c_uint64)]),
+    "lg_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(LgResult)]),
+    "lg_server_get_stats": (C.c_int, [_P, C.POINTER(LgServerStats)]),
 }
 
 _lib = None

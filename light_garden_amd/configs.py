@@ -217,3 +217,13 @@ def batched_uneven(n_scenes: int = 256, seed: int = 4, ray_scale: float = 1.0) -
         scenes.append(Scene(objs, lights, max_bounce=max_b, width=side, height=side,
                             name="batched_%d" % i))
     return scenes
+
+
+def edit_object(obj: Object, rng: SplitMix64) -> Object:
+    """A serving edit: the object moved by up to 0.05 in x and y, or resized by a factor in
+    [0.8, 1.2] about its own centre (even draw: move, odd: resize)."""
+    if rng.randint(0, 1) == 0:
+        geo = obj.geometry.translated(rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05))
+    else:
+        geo = obj.geometry.scaled(rng.uniform(0.8, 1.2))
+    return Object(geo, obj.material_opt)

@@ -276,9 +276,9 @@ static int upload(DevBuf& b, const U* data, size_t n, hipStream_t st) {
   return LG_OK;
 }
 
-int compile_scene(lg_scene* sc) {
+// Uploads on `st`, the stream of the frame that will read the scene.
+int compile_scene(lg_scene* sc, hipStream_t st) {
   if (!sc->dirty) return LG_OK;
-  hipStream_t st = sc->ctx->stream;
   std::vector<Rec<double>> recs;
   std::vector<double> verts;
   std::vector<int32_t> obj_rec;
@@ -470,9 +470,13 @@ int lg_ctx_destroy(lg_ctx* ctx) {
   return LG_OK;
 }
 
-static void ctx_child_gone(lg_ctx* ctx) {
+}  // extern "C"
+
+void lg::ctx_child_gone(lg_ctx* ctx) {
   if (--ctx->children == 0 && ctx->closing) ctx_free(ctx);
 }
+
+extern "C" {
 
 int lg_ctx_synchronize(lg_ctx* ctx) {
   if (!ctx) return fail(LG_E_INVALID, "ctx is NULL");

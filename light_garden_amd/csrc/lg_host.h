@@ -2,6 +2,7 @@
 // scene compilation) and lg_pipeline.hip (the batched frame pipeline).
 #pragma once
 
+#include <atomic>
 #include <memory>
 #include <string>
 #include <vector>
@@ -109,7 +110,7 @@ struct lg_ctx {
   int device = 0;
   int n_cu = 256;
   hipStream_t stream = nullptr;
-  int children = 0;      // live scenes, segment buffers and images
+  std::atomic<int> children{0};  // live scenes, segment buffers, images and servers (any thread)
   bool closing = false;  // lg_ctx_destroy was called while children were alive
   bool profiling = false;
   bool use_graphs = true;
@@ -174,16 +175,30 @@ struct Entry {
   lg_image* img = nullptr;
 };
 
-int compile_scene(lg_scene* sc);
+int compile_scene(lg_scene* sc, hipStream_t st);
 int seg_ensure(lg_segments* s, uint64_t cap, bool need64);
 AccumView make_view(const lg_view& v);
 int check_params(const lg_params* p);
+// A scene, segment buffer, image or server of ctx was destroyed (frees ctx when it was
+// closed and this was its last child).
+void ctx_child_gone(lg_ctx* ctx);
 void begin_call(lg_ctx* ctx);
 void end_call(lg_ctx* ctx);
 
 // Runs `stages` for every entry as one batched frame on ctx's stream and waits for it.
 int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& entries, uint32_t stages);
 void pipeline_destroy(lg_ctx* ctx);
+
+// A batched frame in flight on a stream of its own, with its own scratch and graph cache
+// (the serving loop keeps several). start() validates and launches; poll() returns with
+// *finished once the frame is complete (relaunching it when a buffer overflowed).
+struct FrameJob;
+FrameJob* frame_job_new(lg_ctx* ctx);
+void frame_job_free(FrameJob* job);
+int frame_job_start(FrameJob* job, std::vector<Entry>&& entries, uint32_t stages);
+int frame_job_poll(FrameJob* job, bool wait, bool* finished);
+const std::vector<Entry>& frame_job_entries(const FrameJob* job);
+size_t frame_job_scratch_bytes(const FrameJob* job);
 
 struct StageScope {
   lg_ctx* ctx;

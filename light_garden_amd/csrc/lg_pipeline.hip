@@ -87,8 +87,18 @@ struct Layout {
   std::vector<uint64_t> px_base;
 };
 
-static int enqueue(lg_ctx* ctx, Pipeline& P, const std::vector<Entry>& E, const Layout& L, uint32_t stages) {
-  hipStream_t st = ctx->stream;
+// Stage events only where the caller asked for them (the synchronous entry points on the
+// context's own stream).
+struct Scope {
+  StageScope* s = nullptr;
+  Scope(lg_ctx* ctx, bool on, const char* name, uint64_t work) {
+    if (on) s = new StageScope(ctx, name, work);
+  }
+  ~Scope() { delete s; }
+};
+
+static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const std::vector<Entry>& E,
+                   const Layout& L, uint32_t stages) {
   const int n = L.n;
   SceneDesc* d_descs = P.descs.as<SceneDesc>();
   BatchCtl* d_ctl = P.ctl.as<BatchCtl>();
@@ -102,11 +112,11 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, const std::vector<Entry>& E, const 
   LG_HIP(launch_fill_u32(P.counters.as<uint32_t>(), n * kCounterStride / 4, 0u, st));
   LG_HIP(launch_fill_u32(P.ctl.as<uint32_t>(), sizeof(BatchCtl) / 4, 0u, st));
   if (stages & ST_CLEAR) {
-    StageScope s(ctx, "clear", L.n_px);
+    Scope s(ctx, profile, "clear", L.n_px);
     LG_HIP(launch_pixels(0, d_descs, P.px_pre.as<uint64_t>(), n, L.n_px, st));
   }
   if (stages & ST_TRACE) {
-    StageScope s(ctx, L.fp64 ? "trace_fp64" : "trace", L.max_bounce);
+    Scope s(ctx, profile, L.fp64 ? "trace_fp64" : "trace", L.max_bounce);
     const int grid = ctx->n_cu * (int)ctx->tune.trace_waves;
     for (uint32_t g = 0; g < L.max_bounce; g++) {
       uint32_t* pre = P.item_pre.as<uint32_t>() + (size_t)g * (n + 1);
@@ -122,7 +132,7 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, const std::vector<Entry>& E, const 
   if (stages & ST_ACCUM) {
     const int grid = ctx->n_cu * (int)kBinWaves;
     {
-      StageScope s(ctx, "bin_count", 0);
+      Scope s(ctx, profile, "bin_count", 0);
       LG_HIP(launch_build_chunks(d_descs, n, kChunkMin, L.bin_target, P.chunk_pre.as<uint32_t>(),
                                  P.cbase.as<uint32_t>(), d_ctl, st));
       LG_HIP(launch_fill_u32(P.counts.as<uint32_t>(), L.n_counts, 0u, st));
@@ -130,7 +140,7 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, const std::vector<Entry>& E, const 
                         P.counts.as<uint32_t>(), nullptr, 0, L.max_tiles, grid, st));
     }
     {
-      StageScope s(ctx, "bin_scan", L.n_counts);
+      Scope s(ctx, profile, "bin_scan", L.n_counts);
       LG_HIP(launch_sum_u64(P.counts.as<uint32_t>(), L.n_counts, d_ctl, st));
       LG_HIP(launch_scan_exclusive(P.counts.as<uint32_t>(), P.offsets.as<uint32_t>(), L.n_counts,
                                    P.scan_scratch.as<uint32_t>(), st));
@@ -141,23 +151,23 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, const std::vector<Entry>& E, const 
                                P.pairs_cap, ctx->tune.min_piece, ctx->tune.max_piece, (uint32_t)ctx->n_cu, st));
     }
     {
-      StageScope s(ctx, "bin_fill", 0);
+      Scope s(ctx, profile, "bin_fill", 0);
       LG_HIP(launch_bin(1, d_descs, n, P.chunk_pre.as<uint32_t>(), P.cbase.as<uint32_t>(), d_ctl,
                         P.offsets.as<uint32_t>(), P.pairs.as<uint32_t>(), P.pairs_cap, L.max_tiles, grid, st));
     }
     {
-      StageScope s(ctx, "tile_accum", 0);
+      Scope s(ctx, profile, "tile_accum", 0);
       LG_HIP(launch_tile_accum(d_descs, P.pairs.as<uint32_t>(), P.items.as<WorkItem>(), P.tile_split.as<uint32_t>(),
                                P.split_fx.as<unsigned long long>(), d_ctl, ctx->n_cu, st));
     }
     {
-      StageScope s(ctx, "tile_merge", 0);
+      Scope s(ctx, profile, "tile_merge", 0);
       LG_HIP(launch_tile_merge(d_descs, P.tile_scene.as<uint32_t>(), P.split_tiles.as<uint32_t>(), d_ctl,
                                P.split_fx.as<unsigned long long>(), P.split_cap, ctx->n_cu, st));
     }
   }
   if (stages & ST_RESOLVE) {
-    StageScope s(ctx, "resolve", L.n_px);
+    Scope s(ctx, profile, "resolve", L.n_px);
     LG_HIP(launch_pixels(1, d_descs, P.px_pre.as<uint64_t>(), n, L.n_px, st));
   }
   char* out = P.stage_out.as<char>();
@@ -166,19 +176,24 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, const std::vector<Entry>& E, const 
   return LG_OK;
 }
 
-int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
-  if (!ctx->pipe) ctx->pipe.reset(new Pipeline());
-  Pipeline& P = *ctx->pipe;
+// One batched frame: its entries, its stages and the layout of its current attempt.
+struct Batch {
+  std::vector<Entry> E;
+  uint32_t stages = 0;
+  Layout L;
+  size_t first_stage = 0;  // first stage event of the current attempt
+};
+
+static int batch_validate(lg_ctx* ctx, hipStream_t st, Batch& B) {
+  const std::vector<Entry>& E = B.E;
+  const uint32_t stages = B.stages;
   const int n = (int)E.size();
-  if (n == 0) return LG_OK;
-  hipStream_t st = ctx->stream;
   const bool trace = stages & ST_TRACE, accum = stages & ST_ACCUM;
   const bool pixels = stages & (ST_CLEAR | ST_RESOLVE | ST_ACCUM);
-
-  // ---- validate ----
   int rc;
-  Layout L;
+  Layout& L = B.L;
   L.n = n;
+  // ---- validate ----
   for (int i = 0; i < n; i++) {
     const Entry& e = E[i];
     if (trace) {
@@ -188,7 +203,7 @@ int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
       bool f = (e.params.flags & LG_TRACE_FP64) != 0;
       if (i == 0) L.fp64 = f;
       if (f != L.fp64) return fail(LG_E_INVALID, "batch: mixed FP32/FP64 traces");
-      if ((rc = compile_scene(e.scene))) return rc;
+      if ((rc = compile_scene(e.scene, st))) return rc;
     }
     if (accum && !e.segs) return fail(LG_E_INVALID, "batch: accumulation needs a segment buffer");
     if (pixels && !e.img) return fail(LG_E_INVALID, "batch: an image is needed");
@@ -199,266 +214,394 @@ int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
     }
   }
 
-  for (int attempt = 0; attempt < 8; attempt++) {
-    const size_t first_stage = ctx->pending.size();
-    // ---- capacities ----
-    L.max_bounce = 0;
-    L.cap_recs = L.cap_objs = L.cap_verts = 1;
-    for (const Entry& e : E) {
-      if (trace) {
-        lg_scene* sc = e.scene;
-        uint64_t qn = std::max<uint64_t>(sc->q_cap, std::min<uint64_t>(2 * sc->n_primary, 0xffffffffull));
-        qn = std::max<uint64_t>(qn, 1);
-        if ((rc = sc->q0.ensure(qn * sizeof(QRay<double>)))) return rc;
-        if ((rc = sc->q1.ensure(qn * sizeof(QRay<double>)))) return rc;
-        sc->q_cap = qn;
-        uint64_t need = std::max<uint64_t>(sc->seg_hint + sc->seg_hint / 8,
-                                           sc->n_primary * std::min<uint32_t>(e.params.max_bounce, 4u) + 1024);
-        need = std::max<uint64_t>(need, e.segs->cap);
-        if ((rc = seg_ensure(e.segs, need, L.fp64))) return rc;
-        L.max_bounce = std::max(L.max_bounce, e.params.max_bounce);
-        L.cap_recs = std::max(L.cap_recs, sc->n_recs);
-        L.cap_objs = std::max(L.cap_objs, sc->n_objs);
-        L.cap_verts = std::max(L.cap_verts, sc->n_verts);
-      }
-    }
+  return LG_OK;
+}
+
+// Sizes every buffer for the attempt, writes the descriptors and launches the frame on `st`
+// (graph replay when its shape is known). Does not wait.
+static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, bool allow_graph, Batch& B) {
+  const std::vector<Entry>& E = B.E;
+  const uint32_t stages = B.stages;
+  const int n = (int)E.size();
+  const bool trace = stages & ST_TRACE, accum = stages & ST_ACCUM;
+  int rc;
+  Layout& L = B.L;
+  // ---- capacities ----
+  L.max_bounce = 0;
+  L.cap_recs = L.cap_objs = L.cap_verts = 1;
+  for (const Entry& e : E) {
     if (trace) {
-      size_t lds = L.fp64 ? trace_lds_bytes<double>(L.cap_recs, L.cap_objs, L.cap_verts)
-                          : trace_lds_bytes<float>(L.cap_recs, L.cap_objs, L.cap_verts);
-      if (lds > 64 * 1024) return fail(LG_E_UNSUPPORTED, "scene too large for the LDS-resident trace");
+      lg_scene* sc = e.scene;
+      uint64_t qn = std::max<uint64_t>(sc->q_cap, std::min<uint64_t>(2 * sc->n_primary, 0xffffffffull));
+      qn = std::max<uint64_t>(qn, 1);
+      if ((rc = sc->q0.ensure(qn * sizeof(QRay<double>)))) return rc;
+      if ((rc = sc->q1.ensure(qn * sizeof(QRay<double>)))) return rc;
+      sc->q_cap = qn;
+      uint64_t need = std::max<uint64_t>(sc->seg_hint + sc->seg_hint / 8,
+                                         sc->n_primary * std::min<uint32_t>(e.params.max_bounce, 4u) + 1024);
+      need = std::max<uint64_t>(need, e.segs->cap);
+      if ((rc = seg_ensure(e.segs, need, L.fp64))) return rc;
+      L.max_bounce = std::max(L.max_bounce, e.params.max_bounce);
+      L.cap_recs = std::max(L.cap_recs, sc->n_recs);
+      L.cap_objs = std::max(L.cap_objs, sc->n_objs);
+      L.cap_verts = std::max(L.cap_verts, sc->n_verts);
     }
-    // ---- layout ----
-    L.tile_base.assign(n, 0);
-    L.px_base.assign(n, 0);
-    L.n_tiles = 0;
-    L.max_tiles = 1;
-    L.n_counts = 0;
-    L.n_px = 0;
-    L.bin_target = 2u * (uint32_t)ctx->n_cu * kBinWaves;
-    for (int i = 0; i < n; i++) {
-      const Entry& e = E[i];
-      if (e.img) {
-        L.px_base[i] = L.n_px;
-        L.n_px += (uint64_t)e.img->view.width * e.img->view.height;
-      }
-      if (accum) {
-        AccumView v = make_view(e.img->view);
-        int t = v.tiles_x * v.tiles_y;
-        if (t > 16384) return fail(LG_E_UNSUPPORTED, "accumulate: image above 8192x8192");
-        L.tile_base[i] = L.n_tiles;
-        L.n_tiles += t;
-        L.max_tiles = std::max(L.max_tiles, t);
+  }
+  if (trace) {
+    size_t lds = L.fp64 ? trace_lds_bytes<double>(L.cap_recs, L.cap_objs, L.cap_verts)
+                        : trace_lds_bytes<float>(L.cap_recs, L.cap_objs, L.cap_verts);
+    if (lds > 64 * 1024) return fail(LG_E_UNSUPPORTED, "scene too large for the LDS-resident trace");
+  }
+  // ---- layout ----
+  L.tile_base.assign(n, 0);
+  L.px_base.assign(n, 0);
+  L.n_tiles = 0;
+  L.max_tiles = 1;
+  L.n_counts = 0;
+  L.n_px = 0;
+  L.bin_target = 2u * (uint32_t)ctx->n_cu * kBinWaves;
+  for (int i = 0; i < n; i++) {
+    const Entry& e = E[i];
+    if (e.img) {
+      L.px_base[i] = L.n_px;
+      L.n_px += (uint64_t)e.img->view.width * e.img->view.height;
+    }
+    if (accum) {
+      AccumView v = make_view(e.img->view);
+      int t = v.tiles_x * v.tiles_y;
+      if (t > 16384) return fail(LG_E_UNSUPPORTED, "accumulate: image above 8192x8192");
+      L.tile_base[i] = L.n_tiles;
+      L.n_tiles += t;
+      L.max_tiles = std::max(L.max_tiles, t);
+    }
+  }
+  // chunks hold at least total / bin_target segments, so a scene has at most
+  // n_s * bin_target / total + 1 of them and the batch fewer than bin_target + n; the matrix
+  // has one more entry so that offsets[count] (the end) is always inside it
+  if (accum) L.n_counts = (uint64_t)L.max_tiles * (L.bin_target + (uint64_t)n) + 1;
+  if (L.n_counts > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "batch too large for one frame");
+  // ---- scratch ----
+  // pieces hold at least min_piece pairs, so a frame without pair overflow has fewer than
+  // n_tiles + pairs_cap / min_piece items
+  const uint32_t items_need =
+      (uint32_t)std::min<uint64_t>(0xffffffffull, (uint64_t)L.n_tiles + P.pairs_cap / ctx->tune.min_piece + 1024);
+  P.items_cap = std::max(P.items_cap, items_need);
+  if ((rc = P.descs.ensure(n * sizeof(SceneDesc)))) return rc;
+  if ((rc = P.counters.ensure(n * kCounterStride))) return rc;
+  if ((rc = P.ctl.ensure(sizeof(BatchCtl)))) return rc;
+  if ((rc = P.item_pre.ensure((size_t)kMaxGen * (n + 1) * 4))) return rc;
+  if ((rc = P.chunk_pre.ensure((size_t)(n + 1) * 4))) return rc;
+  if ((rc = P.cbase.ensure((size_t)(n + 1) * 4))) return rc;
+  if ((rc = P.counts.ensure(std::max<uint64_t>(L.n_counts, 1) * 4))) return rc;
+  if ((rc = P.offsets.ensure(std::max<uint64_t>(L.n_counts, 1) * 4))) return rc;
+  uint64_t ss = scan_scratch_elems(L.n_counts);
+  if ((rc = P.scan_scratch.ensure(ss * 4))) return rc;
+  if ((rc = P.tile_scene.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
+  if ((rc = P.tile_start.ensure((size_t)(L.n_tiles + 1) * 4))) return rc;
+  if ((rc = P.pieces.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
+  if ((rc = P.pairs.ensure(P.pairs_cap * 4))) return rc;
+  if ((rc = P.items.ensure((size_t)P.items_cap * sizeof(WorkItem)))) return rc;
+  if ((rc = P.px_pre.ensure((size_t)n * 8))) return rc;
+  {
+    uint64_t sc = std::max<uint64_t>(2ull * ctx->n_cu + 64, P.split_min);
+    sc = std::min<uint64_t>(sc, (uint64_t)std::max(L.n_tiles, 1));
+    P.split_cap = (uint32_t)sc;
+    if ((rc = P.tile_split.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
+    if ((rc = P.split_tiles.ensure((size_t)P.split_cap * 4))) return rc;
+    const size_t fx_bytes = (size_t)P.split_cap * merge_slot_u64() * 8;
+    if (P.split_fx.bytes < fx_bytes) {
+      // slots start zeroed; tile_merge leaves every slot it reads zeroed again
+      if ((rc = P.split_fx.ensure(fx_bytes))) return rc;
+      LG_HIP(launch_fill_u32(P.split_fx.as<uint32_t>(), P.split_fx.bytes / 4, 0u, st));
+      LG_HIP(hipStreamSynchronize(st));
+    }
+  }
+  const size_t desc_bytes = n * sizeof(SceneDesc), tile_bytes = (size_t)L.n_tiles * 4;
+  if ((rc = P.stage_in.ensure(desc_bytes + tile_bytes + (size_t)n * 8))) return rc;
+  if ((rc = P.stage_out.ensure(n * kCounterStride + sizeof(BatchCtl)))) return rc;
+  // ---- descriptors ----
+  char* in = P.stage_in.as<char>();
+  SceneDesc* hd = reinterpret_cast<SceneDesc*>(in);
+  uint32_t* h_tile_scene = reinterpret_cast<uint32_t*>(in + desc_bytes);
+  uint64_t* h_px = reinterpret_cast<uint64_t*>(in + desc_bytes + tile_bytes);
+  for (int i = 0; i < n; i++) {
+    const Entry& e = E[i];
+    SceneDesc& D = hd[i];
+    std::memset(&D, 0, sizeof(D));
+    D.seg_count = gbl(reinterpret_cast<unsigned long long*>(P.counters.as<char>() + i * kCounterStride));
+    D.gen_count = gbl(reinterpret_cast<uint32_t*>(P.counters.as<char>() + i * kCounterStride + 8));
+    float max_abs = 0.0f;
+    if (trace) {
+      lg_scene* sc = e.scene;
+      D.recs = gbl(L.fp64 ? sc->recs64.p : sc->recs32.p);
+      D.recs64 = gbl(sc->recs64.as<const Rec<double>>());
+      D.obj_n64 = gbl(sc->obj_n64.as<const double>());
+      D.obj_flags = gbl(sc->obj_flags.as<const uint32_t>());
+      D.obj_rec = gbl(sc->obj_rec.as<const int32_t>());
+      D.verts = gbl(L.fp64 ? sc->verts64.p : sc->verts32.p);
+      D.verts64 = gbl(sc->verts64.as<const double>());
+      D.lights = gbl(sc->lights_dev.as<const DevLight>());
+      D.n_recs = sc->n_recs;
+      D.n_objs = sc->n_objs;
+      D.n_verts = sc->n_verts;
+      D.n_lights = (int32_t)sc->lights.size();
+      D.n_primary = sc->n_primary;
+      D.max_bounce = e.params.max_bounce;
+      for (int c = 0; c < 4; c++) D.cutoff[c] = e.params.cutoff_color[c];
+      for (int c = 0; c < 4; c++) D.canvas[c] = e.params.canvas[c];
+      D.q0 = gbl(sc->q0.as<QRay<double>>());
+      D.q1 = gbl(sc->q1.as<QRay<double>>());
+      D.q_cap = sc->q_cap;
+      max_abs = sc->light_max;
+    }
+    if (e.segs) {
+      D.seg_pos = gbl(e.segs->pos.as<float4>());
+      D.seg_col = gbl(e.segs->col0.as<float4>());
+      D.seg_col1 = gbl((e.segs->flags & LG_SEG_VERTEX_COLORS) ? e.segs->col1.as<const float4>() : nullptr);
+      D.seg_key = gbl(e.segs->key.as<uint64_t>());
+      D.seg_pos64 = gbl((trace && L.fp64) ? e.segs->pos64.as<double>() : nullptr);
+      D.seg_cap = e.segs->cap;
+      if (!trace) {
+        D.flags |= SCENE_FIXED_COUNT;
+        D.seg_fixed = e.segs->count;
+        max_abs = e.segs->max_abs;
       }
     }
-    // chunks hold at least total / bin_target segments, so a scene has at most
-    // n_s * bin_target / total + 1 of them and the batch fewer than bin_target + n; the matrix
-    // has one more entry so that offsets[count] (the end) is always inside it
-    if (accum) L.n_counts = (uint64_t)L.max_tiles * (L.bin_target + (uint64_t)n) + 1;
-    if (L.n_counts > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "batch too large for one frame");
-    // ---- scratch ----
-    // pieces hold at least min_piece pairs, so a frame without pair overflow has fewer than
-    // n_tiles + pairs_cap / min_piece items
-    const uint32_t items_need =
-        (uint32_t)std::min<uint64_t>(0xffffffffull, (uint64_t)L.n_tiles + P.pairs_cap / ctx->tune.min_piece + 1024);
-    P.items_cap = std::max(P.items_cap, items_need);
-    if ((rc = P.descs.ensure(n * sizeof(SceneDesc)))) return rc;
-    if ((rc = P.counters.ensure(n * kCounterStride))) return rc;
-    if ((rc = P.ctl.ensure(sizeof(BatchCtl)))) return rc;
-    if ((rc = P.item_pre.ensure((size_t)kMaxGen * (n + 1) * 4))) return rc;
-    if ((rc = P.chunk_pre.ensure((size_t)(n + 1) * 4))) return rc;
-    if ((rc = P.cbase.ensure((size_t)(n + 1) * 4))) return rc;
-    if ((rc = P.counts.ensure(std::max<uint64_t>(L.n_counts, 1) * 4))) return rc;
-    if ((rc = P.offsets.ensure(std::max<uint64_t>(L.n_counts, 1) * 4))) return rc;
-    uint64_t ss = scan_scratch_elems(L.n_counts);
-    if ((rc = P.scan_scratch.ensure(ss * 4))) return rc;
-    if ((rc = P.tile_scene.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
-    if ((rc = P.tile_start.ensure((size_t)(L.n_tiles + 1) * 4))) return rc;
-    if ((rc = P.pieces.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
-    if ((rc = P.pairs.ensure(P.pairs_cap * 4))) return rc;
-    if ((rc = P.items.ensure((size_t)P.items_cap * sizeof(WorkItem)))) return rc;
-    if ((rc = P.px_pre.ensure((size_t)n * 8))) return rc;
-    {
-      uint64_t sc = std::max<uint64_t>(2ull * ctx->n_cu + 64, P.split_min);
-      sc = std::min<uint64_t>(sc, (uint64_t)std::max(L.n_tiles, 1));
-      P.split_cap = (uint32_t)sc;
-      if ((rc = P.tile_split.ensure((size_t)std::max(L.n_tiles, 1) * 4))) return rc;
-      if ((rc = P.split_tiles.ensure((size_t)P.split_cap * 4))) return rc;
-      const size_t fx_bytes = (size_t)P.split_cap * merge_slot_u64() * 8;
-      if (P.split_fx.bytes < fx_bytes) {
-        // slots start zeroed; tile_merge leaves every slot it reads zeroed again
-        if ((rc = P.split_fx.ensure(fx_bytes))) return rc;
-        LG_HIP(launch_fill_u32(P.split_fx.as<uint32_t>(), P.split_fx.bytes / 4, 0u, st));
-        LG_HIP(hipStreamSynchronize(st));
-      }
-    }
-    const size_t desc_bytes = n * sizeof(SceneDesc), tile_bytes = (size_t)L.n_tiles * 4;
-    if ((rc = P.stage_in.ensure(desc_bytes + tile_bytes + (size_t)n * 8))) return rc;
-    if ((rc = P.stage_out.ensure(n * kCounterStride + sizeof(BatchCtl)))) return rc;
-    // ---- descriptors ----
-    char* in = P.stage_in.as<char>();
-    SceneDesc* hd = reinterpret_cast<SceneDesc*>(in);
-    uint32_t* h_tile_scene = reinterpret_cast<uint32_t*>(in + desc_bytes);
-    uint64_t* h_px = reinterpret_cast<uint64_t*>(in + desc_bytes + tile_bytes);
-    for (int i = 0; i < n; i++) {
-      const Entry& e = E[i];
-      SceneDesc& D = hd[i];
-      std::memset(&D, 0, sizeof(D));
-      D.seg_count = gbl(reinterpret_cast<unsigned long long*>(P.counters.as<char>() + i * kCounterStride));
-      D.gen_count = gbl(reinterpret_cast<uint32_t*>(P.counters.as<char>() + i * kCounterStride + 8));
-      float max_abs = 0.0f;
-      if (trace) {
-        lg_scene* sc = e.scene;
-        D.recs = gbl(L.fp64 ? sc->recs64.p : sc->recs32.p);
-        D.recs64 = gbl(sc->recs64.as<const Rec<double>>());
-        D.obj_n64 = gbl(sc->obj_n64.as<const double>());
-        D.obj_flags = gbl(sc->obj_flags.as<const uint32_t>());
-        D.obj_rec = gbl(sc->obj_rec.as<const int32_t>());
-        D.verts = gbl(L.fp64 ? sc->verts64.p : sc->verts32.p);
-        D.verts64 = gbl(sc->verts64.as<const double>());
-        D.lights = gbl(sc->lights_dev.as<const DevLight>());
-        D.n_recs = sc->n_recs;
-        D.n_objs = sc->n_objs;
-        D.n_verts = sc->n_verts;
-        D.n_lights = (int32_t)sc->lights.size();
-        D.n_primary = sc->n_primary;
-        D.max_bounce = e.params.max_bounce;
-        for (int c = 0; c < 4; c++) D.cutoff[c] = e.params.cutoff_color[c];
-        for (int c = 0; c < 4; c++) D.canvas[c] = e.params.canvas[c];
-        D.q0 = gbl(sc->q0.as<QRay<double>>());
-        D.q1 = gbl(sc->q1.as<QRay<double>>());
-        D.q_cap = sc->q_cap;
-        max_abs = sc->light_max;
-      }
-      if (e.segs) {
-        D.seg_pos = gbl(e.segs->pos.as<float4>());
-        D.seg_col = gbl(e.segs->col0.as<float4>());
-        D.seg_col1 = gbl((e.segs->flags & LG_SEG_VERTEX_COLORS) ? e.segs->col1.as<const float4>() : nullptr);
-        D.seg_key = gbl(e.segs->key.as<uint64_t>());
-        D.seg_pos64 = gbl((trace && L.fp64) ? e.segs->pos64.as<double>() : nullptr);
-        D.seg_cap = e.segs->cap;
-        if (!trace) {
-          D.flags |= SCENE_FIXED_COUNT;
-          D.seg_fixed = e.segs->count;
-          max_abs = e.segs->max_abs;
-        }
-      }
-      if (e.img) {
-        D.view = make_view(e.img->view);
-        D.accum = gbl(e.img->accum.as<float4>());
-        D.out16 = gbl(e.img->out16.as<ushort4>());
-        D.px_base = L.px_base[i];
-        h_px[i] = L.px_base[i];
-      } else {
-        h_px[i] = L.n_px;
-      }
-      D.color_max = max_abs;
-      D.tile_base = L.tile_base[i];
-      if (accum) {
-        int t = D.view.tiles_x * D.view.tiles_y;
-        for (int k = 0; k < t; k++) h_tile_scene[L.tile_base[i] + k] = (uint32_t)i;
-      }
-    }
-    // ---- launch (graph replay when the shape is unchanged) ----
-    std::vector<uint64_t> shape = {(uint64_t)n, stages, L.fp64, L.max_bounce, (uint64_t)L.cap_recs,
-                                   (uint64_t)L.cap_objs, (uint64_t)L.cap_verts, (uint64_t)L.n_tiles,
-                                   (uint64_t)L.max_tiles, L.n_counts, L.n_px, L.bin_target, P.pairs_cap, P.items_cap,
-                                   P.split_cap, ctx->tune.rays_per_item, ctx->tune.min_piece, ctx->tune.max_piece,
-                                   ctx->tune.trace_waves};
-    for (DevBuf* b : {&P.descs, &P.counters, &P.ctl, &P.item_pre, &P.chunk_pre, &P.cbase, &P.counts, &P.offsets,
-                      &P.scan_scratch, &P.tile_scene, &P.tile_start, &P.pieces, &P.pairs, &P.items,
-                      &P.px_pre, &P.tile_split, &P.split_tiles, &P.split_fx})
-      shape.push_back((uint64_t)b->p);
-    shape.push_back((uint64_t)P.stage_in.p);
-    shape.push_back((uint64_t)P.stage_out.p);
-    const bool use_graph = ctx->use_graphs && !ctx->profiling;
-    if (use_graph) {
-      Pipeline::Graph* G = P.find(shape);
-      if (!G) {
-        P.make_room();
-        LG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = enqueue(ctx, P, E, L, stages);
-        hipGraph_t g = nullptr;
-        hipError_t ce = hipStreamEndCapture(st, &g);
-        if (rc) {
-          if (g) (void)hipGraphDestroy(g);
-          return rc;
-        }
-        if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
-        Pipeline::Graph ng;
-        ng.shape = shape;
-        ng.graph = g;
-        hipError_t ie = hipGraphInstantiate(&ng.exec, g, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-          (void)hipGraphDestroy(g);
-          return hip_fail(ie, "hipGraphInstantiate");
-        }
-        P.graphs.push_back(ng);
-        G = &P.graphs.back();
-      }
-      G->used = ++P.tick;
-      LG_HIP(hipGraphLaunch(G->exec, st));
+    if (e.img) {
+      D.view = make_view(e.img->view);
+      D.accum = gbl(e.img->accum.as<float4>());
+      D.out16 = gbl(e.img->out16.as<ushort4>());
+      D.px_base = L.px_base[i];
+      h_px[i] = L.px_base[i];
     } else {
-      if ((rc = enqueue(ctx, P, E, L, stages))) return rc;
+      h_px[i] = L.n_px;
     }
-    LG_HIP(hipStreamSynchronize(st));
-    // ---- overflow check ----
-    const char* out = P.stage_out.as<const char>();
-    const BatchCtl* hc = reinterpret_cast<const BatchCtl*>(out + n * kCounterStride);
-    if (ctx->profiling) {  // device-side work counts of this attempt's stages
-      uint64_t segs_total = 0;
-      for (int i = 0; trace && i < n; i++) segs_total += *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
-      for (size_t k = first_stage; k < ctx->pending.size(); k++) {
-        StageRec& r = ctx->pending[k];
-        if (r.name.compare(0, 5, "trace") == 0) r.work = segs_total;
-        if (r.name == "bin_count" || r.name == "bin_fill") r.work = hc->total_pairs;
-        if (r.name == "tile_accum") r.work = hc->samples;
+    D.color_max = max_abs;
+    D.tile_base = L.tile_base[i];
+    if (accum) {
+      int t = D.view.tiles_x * D.view.tiles_y;
+      for (int k = 0; k < t; k++) h_tile_scene[L.tile_base[i] + k] = (uint32_t)i;
+    }
+  }
+  // ---- launch (graph replay when the shape is unchanged) ----
+  std::vector<uint64_t> shape = {(uint64_t)n, stages, L.fp64, L.max_bounce, (uint64_t)L.cap_recs,
+                                 (uint64_t)L.cap_objs, (uint64_t)L.cap_verts, (uint64_t)L.n_tiles,
+                                 (uint64_t)L.max_tiles, L.n_counts, L.n_px, L.bin_target, P.pairs_cap, P.items_cap,
+                                 P.split_cap, ctx->tune.rays_per_item, ctx->tune.min_piece, ctx->tune.max_piece,
+                                 ctx->tune.trace_waves};
+  for (DevBuf* b : {&P.descs, &P.counters, &P.ctl, &P.item_pre, &P.chunk_pre, &P.cbase, &P.counts, &P.offsets,
+                    &P.scan_scratch, &P.tile_scene, &P.tile_start, &P.pieces, &P.pairs, &P.items,
+                    &P.px_pre, &P.tile_split, &P.split_tiles, &P.split_fx})
+    shape.push_back((uint64_t)b->p);
+  shape.push_back((uint64_t)P.stage_in.p);
+  shape.push_back((uint64_t)P.stage_out.p);
+  const bool use_graph = allow_graph && ctx->use_graphs && !profile;
+  if (use_graph) {
+    Pipeline::Graph* G = P.find(shape);
+    if (!G) {
+      P.make_room();
+      LG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = enqueue(ctx, P, st, profile, E, L, stages);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
       }
+      if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+      Pipeline::Graph ng;
+      ng.shape = shape;
+      ng.graph = g;
+      hipError_t ie = hipGraphInstantiate(&ng.exec, g, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(ie, "hipGraphInstantiate");
+      }
+      P.graphs.push_back(ng);
+      G = &P.graphs.back();
     }
-    bool over = false;
+    G->used = ++P.tick;
+    LG_HIP(hipGraphLaunch(G->exec, st));
+  } else {
+    if ((rc = enqueue(ctx, P, st, profile, E, L, stages))) return rc;
+  }
+  return LG_OK;
+}
+
+// After the frame's stream has completed: grows whatever overflowed (*rerun = true: launch
+// again) or publishes the segment counts.
+static int batch_finish(lg_ctx* ctx, Pipeline& P, bool profile, Batch& B, bool* rerun) {
+  const std::vector<Entry>& E = B.E;
+  const uint32_t stages = B.stages;
+  const int n = (int)E.size();
+  const bool trace = stages & ST_TRACE, accum = stages & ST_ACCUM;
+  const Layout& L = B.L;
+  // ---- overflow check ----
+  const char* out = P.stage_out.as<const char>();
+  const BatchCtl* hc = reinterpret_cast<const BatchCtl*>(out + n * kCounterStride);
+  if (profile) {  // device-side work counts of this attempt's stages
+    uint64_t segs_total = 0;
+    for (int i = 0; trace && i < n; i++) segs_total += *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
+    for (size_t k = B.first_stage; k < ctx->pending.size(); k++) {
+      StageRec& r = ctx->pending[k];
+      if (r.name.compare(0, 5, "trace") == 0) r.work = segs_total;
+      if (r.name == "bin_count" || r.name == "bin_fill") r.work = hc->total_pairs;
+      if (r.name == "tile_accum") r.work = hc->samples;
+    }
+  }
+  bool over = false;
+  for (int i = 0; trace && i < n; i++) {
+    const Entry& e = E[i];
+    const uint64_t segs = *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
+    const uint32_t* gc = reinterpret_cast<const uint32_t*>(out + i * kCounterStride + 8);
+    uint64_t qmax = 0;
+    for (uint32_t g = 0; g < e.params.max_bounce; g++) qmax = std::max<uint64_t>(qmax, gc[g]);
+    if (segs > e.segs->cap) {
+      e.scene->seg_hint = std::max<uint64_t>(e.scene->seg_hint, segs + segs / 4 + 1024);
+      over = true;
+    }
+    if (qmax > e.scene->q_cap) {
+      e.scene->q_cap = std::min<uint64_t>(qmax + qmax / 4 + 1024, 0xffffffffull);
+      if (qmax >= 0xffffffffull) return fail(LG_E_UNSUPPORTED, "ray queue exceeds 2^32 entries");
+      over = true;
+    }
+  }
+  if (accum) {
+    if (hc->total_pairs > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "more than 2^32 segment-tile pairs");
+    if (hc->flags & CTL_PAIRS_OVERFLOW) {
+      P.pairs_cap = hc->total_pairs + hc->total_pairs / 4 + 1024;
+      over = true;
+    }
+    if (hc->flags & CTL_ITEMS_OVERFLOW) {
+      P.items_cap *= 2;
+      over = true;
+    }
+    if (hc->flags & CTL_SPLIT_OVERFLOW) {
+      P.split_min = std::max<uint32_t>(hc->n_split + hc->n_split / 4 + 64, P.split_cap);
+      over = true;
+    }
+  }
+  *rerun = over;
+  if (!over) {
     for (int i = 0; trace && i < n; i++) {
       const Entry& e = E[i];
       const uint64_t segs = *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
-      const uint32_t* gc = reinterpret_cast<const uint32_t*>(out + i * kCounterStride + 8);
-      uint64_t qmax = 0;
-      for (uint32_t g = 0; g < e.params.max_bounce; g++) qmax = std::max<uint64_t>(qmax, gc[g]);
-      if (segs > e.segs->cap) {
-        e.scene->seg_hint = std::max<uint64_t>(e.scene->seg_hint, segs + segs / 4 + 1024);
-        over = true;
-      }
-      if (qmax > e.scene->q_cap) {
-        e.scene->q_cap = std::min<uint64_t>(qmax + qmax / 4 + 1024, 0xffffffffull);
-        if (qmax >= 0xffffffffull) return fail(LG_E_UNSUPPORTED, "ray queue exceeds 2^32 entries");
-        over = true;
-      }
-    }
-    if (accum) {
-      if (hc->total_pairs > 0xffffffffull) return fail(LG_E_UNSUPPORTED, "more than 2^32 segment-tile pairs");
-      if (hc->flags & CTL_PAIRS_OVERFLOW) {
-        P.pairs_cap = hc->total_pairs + hc->total_pairs / 4 + 1024;
-        over = true;
-      }
-      if (hc->flags & CTL_ITEMS_OVERFLOW) {
-        P.items_cap *= 2;
-        over = true;
-      }
-      if (hc->flags & CTL_SPLIT_OVERFLOW) {
-        P.split_min = std::max<uint32_t>(hc->n_split + hc->n_split / 4 + 64, P.split_cap);
-        over = true;
-      }
-    }
-    if (!over) {
-      for (int i = 0; trace && i < n; i++) {
-        const Entry& e = E[i];
-        const uint64_t segs = *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
-        e.segs->count = segs;
-        e.segs->has_pos64 = L.fp64;
-        e.segs->max_abs = e.scene->light_max;
-        e.scene->seg_hint = std::max(e.scene->seg_hint, segs);
-      }
-      return LG_OK;
+      e.segs->count = segs;
+      e.segs->has_pos64 = L.fp64;
+      e.segs->max_abs = e.scene->light_max;
+      e.scene->seg_hint = std::max(e.scene->seg_hint, segs);
     }
   }
+  return LG_OK;
+}
+
+constexpr int kMaxAttempts = 8;
+
+int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& E, uint32_t stages) {
+  if (!ctx->pipe) ctx->pipe.reset(new Pipeline());
+  Pipeline& P = *ctx->pipe;
+  if (E.empty()) return LG_OK;
+  Batch B;
+  B.E = E;
+  B.stages = stages;
+  int rc;
+  if ((rc = batch_validate(ctx, ctx->stream, B))) return rc;
+  for (int attempt = 0; attempt < kMaxAttempts; attempt++) {
+    B.first_stage = ctx->pending.size();
+    if ((rc = batch_launch(ctx, P, ctx->stream, ctx->profiling, true, B))) return rc;
+    LG_HIP(hipStreamSynchronize(ctx->stream));
+    bool rerun = false;
+    if ((rc = batch_finish(ctx, P, ctx->profiling, B, &rerun))) return rc;
+    if (!rerun) return LG_OK;
+  }
   return fail(LG_E_CAPACITY, "batch: buffers kept overflowing");
+}
+
+// ---- asynchronous frames (the serving loop) ----------------------------------------------------
+struct FrameJob {
+  lg_ctx* ctx = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  Pipeline P;
+  Batch B;
+  int attempts = 0;
+  bool busy = false;
+};
+
+FrameJob* frame_job_new(lg_ctx* ctx) {
+  FrameJob* j = new FrameJob();
+  j->ctx = ctx;
+  if (hipStreamCreateWithFlags(&j->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&j->done, hipEventDisableTiming) != hipSuccess) {
+    frame_job_free(j);
+    return nullptr;
+  }
+  return j;
+}
+
+void frame_job_free(FrameJob* j) {
+  if (!j) return;
+  if (j->st) (void)hipStreamSynchronize(j->st);
+  if (j->done) (void)hipEventDestroy(j->done);
+  if (j->st) (void)hipStreamDestroy(j->st);
+  delete j;
+}
+
+static int job_launch(FrameJob* j) {
+  int rc;
+  // eager: the batches of a serving mix rarely repeat a shape, so a capture would not pay
+  if ((rc = batch_launch(j->ctx, j->P, j->st, false, false, j->B))) return rc;
+  LG_HIP(hipEventRecord(j->done, j->st));
+  j->attempts++;
+  return LG_OK;
+}
+
+int frame_job_start(FrameJob* j, std::vector<Entry>&& E, uint32_t stages) {
+  if (j->busy) return fail(LG_E_INVALID, "frame job already running");
+  j->B = Batch();
+  j->B.E = std::move(E);
+  j->B.stages = stages;
+  j->attempts = 0;
+  int rc;
+  if ((rc = batch_validate(j->ctx, j->st, j->B))) return rc;
+  if ((rc = job_launch(j))) return rc;
+  j->busy = true;
+  return LG_OK;
+}
+
+int frame_job_poll(FrameJob* j, bool wait, bool* finished) {
+  *finished = false;
+  if (!j->busy) {
+    *finished = true;
+    return LG_OK;
+  }
+  hipError_t q = wait ? hipEventSynchronize(j->done) : hipEventQuery(j->done);
+  if (q == hipErrorNotReady) return LG_OK;
+  j->busy = false;
+  if (q != hipSuccess) return hip_fail(q, "frame job");
+  bool rerun = false;
+  int rc;
+  if ((rc = batch_finish(j->ctx, j->P, false, j->B, &rerun))) return rc;
+  if (!rerun) {
+    *finished = true;
+    return LG_OK;
+  }
+  if (j->attempts >= kMaxAttempts) return fail(LG_E_CAPACITY, "batch: buffers kept overflowing");
+  if ((rc = job_launch(j))) return rc;
+  j->busy = true;
+  return LG_OK;
+}
+
+const std::vector<Entry>& frame_job_entries(const FrameJob* j) { return j->B.E; }
+
+size_t frame_job_scratch_bytes(const FrameJob* j) {
+  const Pipeline& P = j->P;
+  size_t b = 0;
+  for (const DevBuf* d : {&P.descs, &P.counters, &P.ctl, &P.item_pre, &P.chunk_pre, &P.cbase, &P.counts, &P.offsets,
+                          &P.scan_scratch, &P.tile_scene, &P.tile_start, &P.pieces, &P.pairs, &P.items, &P.px_pre,
+                          &P.tile_split, &P.split_tiles, &P.split_fx})
+    b += d->bytes;
+  return b;
 }
 
 }  // namespace lg

@@ -1,0 +1,379 @@
+// lg_server.hip — serving loop: continuous batching of render requests over resident scenes.
+//
+// Requests (scene, params, image, optional one-object edit) are queued by any thread. A
+// worker thread keeps up to `slots` batched frames in flight, each a FrameJob on its own
+// stream. Whenever a slot is free it forms the next frame from the waiting requests, in
+// arrival order, as long as:
+//   * the request's scene and image are not in a frame already in flight or already in the
+//     frame being formed, and no earlier request for the same scene is still waiting;
+//   * the device memory of the resident working set (every resident scene's ray queues and
+//     segment buffer plus the images of admitted requests, and the frames' scratch) stays
+//     inside the HBM budget: idle resident scenes are evicted least-recently-used first to
+//     make room, and a request waits while the budget is held by frames in flight.
+// So a request that arrives while other frames run starts on the next free slot and runs
+// on the device alongside them. Scene state (geometry, queues, segment buffer) stays
+// resident across requests and edits until evicted. Edits are applied when their request is
+// admitted, so they are ordered with the renders of their scene. A frame's results do not
+// depend on which other scenes share it (the fixed-point accumulation is exact).
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+
+#include "lg_host.h"
+
+using namespace lg;
+
+namespace {
+
+struct Pending {
+  uint64_t ticket;
+  lg_request req;
+  double submit_us;
+};
+
+struct Resident {
+  lg_segments* segs = nullptr;  // server-owned segment buffer of the scene
+  uint64_t last_use = 0;
+  bool busy = false;            // in a frame in flight
+};
+
+struct Slot {
+  FrameJob* job = nullptr;
+  std::vector<Pending> reqs;
+  double start_us = 0.0;
+  bool busy = false;
+};
+
+size_t buf_bytes(const DevBuf& b) { return b.bytes; }
+
+size_t scene_evictable_bytes(const lg_scene* sc, const lg_segments* segs) {
+  size_t b = buf_bytes(sc->q0) + buf_bytes(sc->q1);
+  if (segs) b += buf_bytes(segs->pos) + buf_bytes(segs->col0) + buf_bytes(segs->col1) + buf_bytes(segs->key) +
+                 buf_bytes(segs->pos64);
+  return b;
+}
+
+size_t image_bytes(const lg_image* im) { return buf_bytes(im->accum) + buf_bytes(im->out16); }
+
+// Device memory a render of `sc` will hold once resident: its ray queues and segment
+// buffer, from the scene's own history when it has one, else from its ray count.
+size_t scene_estimate(const lg_scene* sc, const lg_params& p, const lg_segments* segs) {
+  size_t now = scene_evictable_bytes(sc, segs);
+  if (now > 0 && sc->seg_hint > 0) return now;
+  uint64_t rays = 0;
+  for (const lg_light& l : sc->lights) rays += l.num_rays;
+  const uint64_t segs_est = rays * std::min<uint32_t>(std::max<uint32_t>(p.max_bounce, 1u), 4u) + 1024;
+  return std::max(now, (size_t)(2 * 2 * rays * sizeof(QRay<double>) + segs_est * 40));
+}
+
+}  // namespace
+
+struct lg_server {
+  lg_ctx* ctx = nullptr;
+  lg_server_config cfg{};
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_work;  // new requests, stop
+  std::condition_variable cv_done;  // completed requests
+  std::deque<Pending> pending;
+  std::unordered_map<uint64_t, lg_result> done;
+  std::unordered_map<lg_scene*, Resident> resident;  // touched only by the worker
+  std::unordered_set<lg_image*> images_busy;          // worker only
+  std::vector<Slot> slots;
+  uint64_t next_ticket = 1;
+  uint64_t use_clock = 0;
+  bool stopping = false;
+  lg_server_stats stats{};
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double busy_since = -1.0;
+
+  double now_us() const {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  size_t resident_bytes() const {
+    size_t b = 0;
+    for (const auto& kv : resident) b += scene_evictable_bytes(kv.first, kv.second.segs);
+    for (const lg_image* im : images_busy) b += image_bytes(im);
+    for (const Slot& s : slots) b += frame_job_scratch_bytes(s.job);
+    return b;
+  }
+
+  void evict(lg_scene* sc) {
+    auto it = resident.find(sc);
+    if (it == resident.end() || it->second.busy) return;
+    sc->q0.reset();
+    sc->q1.reset();
+    sc->q_cap = 0;
+    if (it->second.segs) lg_segments_destroy(it->second.segs);
+    resident.erase(it);
+    stats.evictions++;
+  }
+
+  // Makes room for `need` more bytes by evicting idle scenes, oldest use first.
+  bool make_room(size_t need, const std::unordered_set<lg_scene*>& keep) {
+    for (;;) {
+      if (resident_bytes() + need <= cfg.hbm_budget) return true;
+      lg_scene* victim = nullptr;
+      uint64_t oldest = UINT64_MAX;
+      for (auto& kv : resident)
+        if (!kv.second.busy && !keep.count(kv.first) && kv.second.last_use < oldest) {
+          oldest = kv.second.last_use;
+          victim = kv.first;
+        }
+      if (!victim) return false;
+      evict(victim);
+    }
+  }
+
+  void complete(Slot& s, int status) {
+    const double end = now_us();
+    const uint32_t n = (uint32_t)s.reqs.size();
+    std::lock_guard<std::mutex> lk(mu);
+    for (Pending& p : s.reqs) {
+      lg_result r{};
+      r.status = status;
+      r.batch_size = n;
+      auto it = resident.find(p.req.scene);
+      if (it != resident.end()) {
+        it->second.busy = false;
+        it->second.last_use = ++use_clock;
+        if (status == LG_OK && it->second.segs) r.segments = it->second.segs->count;
+      }
+      images_busy.erase(p.req.image);
+      r.submit_us = p.submit_us;
+      r.start_us = s.start_us;
+      r.end_us = end;
+      done[p.ticket] = r;
+      stats.completed++;
+    }
+    s.reqs.clear();
+    s.busy = false;
+    cv_done.notify_all();
+  }
+
+  // Forms and launches a frame on an idle slot from the waiting requests. Returns whether a
+  // frame was launched.
+  bool launch(Slot& s) {
+    std::vector<Pending> take;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      std::unordered_set<lg_scene*> blocked, chosen;
+      std::unordered_set<lg_image*> imgs;
+      size_t extra = 0;
+      bool memory_wait = false;
+      for (auto it = pending.begin(); it != pending.end() && take.size() < cfg.max_batch;) {
+        lg_scene* sc = it->req.scene;
+        lg_image* im = it->req.image;
+        auto rs = resident.find(sc);
+        const bool sc_busy = rs != resident.end() && rs->second.busy;
+        if (blocked.count(sc) || chosen.count(sc) || sc_busy || imgs.count(im) || images_busy.count(im)) {
+          blocked.insert(sc);  // keep this scene's later requests behind this one
+          ++it;
+          continue;
+        }
+        const lg_segments* segs = rs != resident.end() ? rs->second.segs : nullptr;
+        size_t need = image_bytes(im) + (rs == resident.end() ? scene_estimate(sc, it->req.params, nullptr)
+                                                              : scene_estimate(sc, it->req.params, segs) -
+                                                                    scene_evictable_bytes(sc, segs));
+        std::unordered_set<lg_scene*> keep = chosen;
+        keep.insert(sc);
+        if (!make_room(extra + need, keep)) {
+          bool anything_running = false;
+          for (const Slot& o : slots) anything_running |= o.busy;
+          if (anything_running || !take.empty()) {
+            memory_wait = true;  // wait for frames in flight to release memory
+            blocked.insert(sc);
+            ++it;
+            continue;
+          }
+          // nothing else holds memory: admit it anyway and let the allocation decide
+        }
+        extra += need;
+        chosen.insert(sc);
+        imgs.insert(im);
+        take.push_back(*it);
+        it = pending.erase(it);
+      }
+      (void)memory_wait;
+      if (take.empty()) return false;
+      for (Pending& p : take) {
+        Resident& r = resident[p.req.scene];
+        r.busy = true;
+        images_busy.insert(p.req.image);
+      }
+    }
+    // outside the lock: edits, buffers, launch
+    std::vector<Entry> E;
+    E.reserve(take.size());
+    int status = LG_OK;
+    for (Pending& p : take) {
+      Resident& r = resident[p.req.scene];
+      if (!r.segs) {
+        int rc = lg_segments_create(ctx, 1024, 0, &r.segs);
+        if (rc) status = rc;
+      }
+      if (p.req.edit_index >= 0) {
+        int rc = lg_scene_replace_object(p.req.scene, (uint32_t)p.req.edit_index, &p.req.edit);
+        if (rc) status = rc;
+      }
+      Entry e;
+      e.scene = p.req.scene;
+      e.params = p.req.params;
+      e.segs = r.segs;
+      e.img = p.req.image;
+      E.push_back(e);
+    }
+    s.reqs = std::move(take);
+    s.start_us = now_us();
+    s.busy = true;
+    stats.frames++;
+    if (status == LG_OK) status = frame_job_start(s.job, std::move(E), ST_CLEAR | ST_TRACE | ST_ACCUM | ST_RESOLVE);
+    if (status != LG_OK) complete(s, status);
+    size_t rb = resident_bytes();
+    stats.resident_bytes = rb;
+    stats.peak_resident_bytes = std::max<uint64_t>(stats.peak_resident_bytes, rb);
+    return true;
+  }
+
+  void run() {
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+      bool progressed = false;
+      for (Slot& s : slots) {
+        if (!s.busy) continue;
+        bool fin = false;
+        int rc = frame_job_poll(s.job, false, &fin);
+        if (rc != LG_OK) {
+          complete(s, rc);
+          progressed = true;
+        } else if (fin) {
+          complete(s, LG_OK);
+          progressed = true;
+        }
+      }
+      bool any_busy = false;
+      for (Slot& s : slots) {
+        if (!s.busy) progressed |= launch(s);
+        any_busy |= s.busy;
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        const double t = now_us();
+        if (any_busy && busy_since < 0) busy_since = t;
+        if (!any_busy && busy_since >= 0) {
+          stats.busy_us += t - busy_since;
+          busy_since = -1.0;
+        }
+      }
+      std::unique_lock<std::mutex> lk(mu);
+      if (stopping && pending.empty() && !any_busy) break;
+      if (progressed) continue;
+      if (any_busy)  // frames in flight: poll their events every few microseconds
+        cv_work.wait_for(lk, std::chrono::microseconds(20));
+      else
+        cv_work.wait_for(lk, std::chrono::milliseconds(5), [&] { return stopping || !pending.empty(); });
+    }
+  }
+};
+
+extern "C" {
+
+int lg_server_create(lg_ctx* ctx, const lg_server_config* cfg, lg_server** out) {
+  if (!ctx || !out) return fail(LG_E_INVALID, "ctx or out is NULL");
+  lg_server_config c{};
+  if (cfg) c = *cfg;
+  if (c.hbm_budget == 0) c.hbm_budget = 64ull << 30;
+  if (c.max_batch == 0) c.max_batch = 256;
+  if (c.slots == 0) c.slots = 2;
+  if (c.slots > 4) return fail(LG_E_INVALID, "server: at most 4 frames in flight");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(LG_E_HIP, "hipSetDevice");
+  lg_server* s = new lg_server();
+  s->ctx = ctx;
+  s->cfg = c;
+  for (uint32_t i = 0; i < c.slots; i++) {
+    Slot sl;
+    sl.job = frame_job_new(ctx);
+    if (!sl.job) {
+      for (Slot& o : s->slots) frame_job_free(o.job);
+      delete s;
+      return fail(LG_E_HIP, "server: stream creation failed");
+    }
+    s->slots.push_back(sl);
+  }
+  ctx->children++;
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return LG_OK;
+}
+
+int lg_server_destroy(lg_server* s) {
+  if (!s) return LG_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  lg_ctx* ctx = s->ctx;
+  for (Slot& sl : s->slots) frame_job_free(sl.job);
+  for (auto& kv : s->resident)
+    if (kv.second.segs) lg_segments_destroy(kv.second.segs);
+  delete s;
+  ctx_child_gone(ctx);
+  return LG_OK;
+}
+
+int lg_server_submit(lg_server* s, const lg_request* req, uint64_t* ticket) {
+  if (!s || !req || !ticket) return fail(LG_E_INVALID, "server, request or ticket is NULL");
+  if (!req->scene || !req->image) return fail(LG_E_INVALID, "request: scene and image are required");
+  if (req->scene->ctx != s->ctx || req->image->ctx != s->ctx) return fail(LG_E_INVALID, "request: other context");
+  int rc;
+  if ((rc = check_params(&req->params))) return rc;
+  if (req->params.flags & LG_TRACE_FP64) return fail(LG_E_INVALID, "server: FP32 traces only");
+  if (req->edit_index >= 0 && (size_t)req->edit_index >= req->scene->objects.size())
+    return fail(LG_E_INVALID, "request: edit index out of range");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return fail(LG_E_INVALID, "server is stopping");
+    Pending p;
+    p.ticket = s->next_ticket++;
+    p.req = *req;
+    p.submit_us = s->now_us();
+    s->pending.push_back(p);
+    s->stats.submitted++;
+    *ticket = p.ticket;
+  }
+  s->cv_work.notify_one();
+  return LG_OK;
+}
+
+int lg_server_wait(lg_server* s, uint64_t ticket, double timeout_s, lg_result* out) {
+  if (!s || !out) return fail(LG_E_INVALID, "server or out is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto ready = [&] { return s->done.count(ticket) > 0; };
+  if (timeout_s < 0) {
+    s->cv_done.wait(lk, ready);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+    return fail(LG_E_TIMEOUT, "request not finished");
+  }
+  *out = s->done[ticket];
+  s->done.erase(ticket);
+  return LG_OK;
+}
+
+int lg_server_get_stats(lg_server* s, lg_server_stats* out) {
+  if (!s || !out) return fail(LG_E_INVALID, "server or out is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->stats;
+  if (s->busy_since >= 0) out->busy_us += s->now_us() - s->busy_since;
+  return LG_OK;
+}
+
+}  // extern "C"

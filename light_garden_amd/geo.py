@@ -14,6 +14,10 @@ Point = Tuple[float, float]
 IDENTITY = (1.0, 0.0, 0.0, 1.0)
 
 
+def _scale_pt(p, c, f):
+    return (c[0] + (p[0] - c[0]) * f, c[1] + (p[1] - c[1]) * f)
+
+
 def rotation(angle: float) -> Tuple[float, float, float, float]:
     c, s = math.cos(angle), math.sin(angle)
     return (c, s, -s, c)
@@ -30,6 +34,11 @@ class LineSegment:
 
     def translated(self, dx, dy):
         return LineSegment((self.a[0] + dx, self.a[1] + dy), (self.b[0] + dx, self.b[1] + dy))
+
+    def scaled(self, f, about=None):
+        """Scaled by f about `about` (default: the midpoint)."""
+        c = about or ((self.a[0] + self.b[0]) * 0.5, (self.a[1] + self.b[1]) * 0.5)
+        return LineSegment(_scale_pt(self.a, c, f), _scale_pt(self.b, c, f))
 
 
 @dataclass
@@ -48,6 +57,12 @@ class CubicBezier:
     def translated(self, dx, dy):
         return CubicBezier([(x + dx, y + dy) for x, y in self.points])
 
+    def scaled(self, f, about=None):
+        """Scaled by f about `about` (default: the centroid of the control points)."""
+        n = len(self.points)
+        c = about or (sum(p[0] for p in self.points) / n, sum(p[1] for p in self.points) / n)
+        return CubicBezier([_scale_pt(p, c, f) for p in self.points])
+
 
 @dataclass
 class Circle:
@@ -59,6 +74,9 @@ class Circle:
 
     def translated(self, dx, dy):
         return Circle((self.origin[0] + dx, self.origin[1] + dy), self.radius)
+
+    def scaled(self, f, about=None):
+        return Circle(_scale_pt(self.origin, about or self.origin, f), self.radius * f)
 
 
 @dataclass
@@ -75,6 +93,9 @@ class Rect:
     def translated(self, dx, dy):
         return Rect((self.origin[0] + dx, self.origin[1] + dy), self.rotation, self.width, self.height)
 
+    def scaled(self, f, about=None):
+        return Rect(_scale_pt(self.origin, about or self.origin, f), self.rotation, self.width * f, self.height * f)
+
 
 @dataclass
 class Ellipse:
@@ -88,6 +109,9 @@ class Ellipse:
 
     def translated(self, dx, dy):
         return Ellipse((self.origin[0] + dx, self.origin[1] + dy), self.a, self.b, self.rotation)
+
+    def scaled(self, f, about=None):
+        return Ellipse(_scale_pt(self.origin, about or self.origin, f), self.a * f, self.b * f, self.rotation)
 
 
 def convex_hull(points: Sequence[Point]) -> List[Point]:
@@ -129,6 +153,10 @@ class ConvexPolygon:
     def translated(self, dx, dy):
         return ConvexPolygon(self.vertices, (self.origin[0] + dx, self.origin[1] + dy), self.rotation)
 
+    def scaled(self, f, about=None):
+        return ConvexPolygon([(x * f, y * f) for x, y in self.vertices],
+                             _scale_pt(self.origin, about or self.origin, f), self.rotation)
+
 
 @dataclass
 class Logic:
@@ -145,6 +173,12 @@ class Logic:
 
     def translated(self, dx, dy):
         return Logic(self.op, self.a, self.b, (self.origin[0] + dx, self.origin[1] + dy), self.rotation)
+
+    def scaled(self, f, about=None):
+        """Both operands scaled by f about the local frame's origin, the frame placed as before
+        (scaled about `about` when given)."""
+        return Logic(self.op, self.a.scaled(f, (0.0, 0.0)), self.b.scaled(f, (0.0, 0.0)),
+                     _scale_pt(self.origin, about or self.origin, f), self.rotation)
 
 
 def lens(origin: Point, radius: float, distance: float) -> Logic:

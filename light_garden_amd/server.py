@@ -1,0 +1,75 @@
+"""Serving loop (lg_server_* in include/lg.h): continuous batching of render requests over
+resident scenes, admitted by an HBM budget. The loop itself is native (a worker thread in
+the library); this module only wraps its C ABI."""
+import ctypes as C
+from typing import Optional
+
+from . import abi
+from .tracer import Context, Image, Object, Tracer
+
+
+class Server:
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_batch: int = 0, slots: int = 0):
+        self.ctx = ctx
+        cfg = abi.LgServerConfig(int(hbm_budget), int(max_batch), int(slots))
+        h = C.c_void_p()
+        abi.call("lg_server_create", ctx.handle, C.byref(cfg), C.byref(h))
+        self._h = h
+        self._live = {}  # ticket -> (tracer, image): kept alive until the request is collected
+        ctx._adopt(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def submit(self, tracer: Tracer, image: Image, edit: Optional[tuple] = None) -> int:
+        """Queues a render of tracer's scene into image; edit = (object index, Object) replaces
+        that object first. Returns the ticket."""
+        tracer.sync()
+        req = abi.LgRequest()
+        req.scene = tracer._h.value
+        req.image = image.handle.value
+        req.params = tracer.params()
+        req.edit_index = -1
+        if edit is not None:
+            ix, obj = edit
+            req.edit_index = int(ix)
+            req.edit = obj.to_abi()
+        t = C.c_uint64()
+        abi.call("lg_server_submit", self._h, C.byref(req), C.byref(t))
+        self._live[t.value] = (tracer, image)
+        if edit is not None:  # keep the Python mirror of the scene in step with the server's edit
+            tracer.objects[int(edit[0])] = edit[1]
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> dict:
+        r = abi.LgResult()
+        abi.call("lg_server_wait", self._h, C.c_uint64(ticket), C.c_double(timeout), C.byref(r))
+        self._live.pop(ticket, None)
+        if r.status != abi.LG_OK:
+            raise abi.LgError(r.status, "server request %d" % ticket, "frame failed")
+        return {"batch_size": r.batch_size, "segments": r.segments, "submit_us": r.submit_us,
+                "start_us": r.start_us, "end_us": r.end_us, "latency_ms": (r.end_us - r.submit_us) / 1e3}
+
+    def stats(self) -> dict:
+        s = abi.LgServerStats()
+        abi.call("lg_server_get_stats", self._h, C.byref(s))
+        return {name: getattr(s, name) for name, _ in abi.LgServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            abi.call("lg_server_destroy", self._h)  # finishes every submitted request
+            self._h = C.c_void_p()
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

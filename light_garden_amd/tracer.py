@@ -211,7 +211,9 @@ class Context:
     def close(self):
         """Closes every live scene/buffer/image of this context, then the context."""
         if self._h:
-            for child in list(self._children):
+            # servers first: they finish their requests, which still use scenes and images
+            kids = list(self._children)
+            for child in sorted(kids, key=lambda c: 0 if type(c).__name__ == "Server" else 1):
                 child.close()
             abi.load().lg_ctx_destroy(self._h)
             self._h = C.c_void_p()

@@ -43,7 +43,8 @@ def test_struct_layout_matches_c():
     fields = {
         "lg_shape": abi.LgShape, "lg_object": abi.LgObject, "lg_light": abi.LgLight,
         "lg_params": abi.LgParams, "lg_view": abi.LgView, "lg_stage_time": abi.LgStageTime,
-        "lg_tuning": abi.LgTuning,
+        "lg_tuning": abi.LgTuning, "lg_server_config": abi.LgServerConfig, "lg_request": abi.LgRequest,
+        "lg_result": abi.LgResult, "lg_server_stats": abi.LgServerStats,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for cname, cls in fields.items():
