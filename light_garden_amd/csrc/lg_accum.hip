@@ -530,7 +530,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
     for (uint32_t base = W.begin + wave * 64; base < W.end; base += kWaves * 64) {
       // ---- per-lane setup of one pair ----
       const uint32_t pi = base + lane;
-      int packed = 0;  // n | q0 << 8 | xmajor << 16 | vertex colours << 17
+      int packed = 0;  // n | q0 << 8 | xmajor << 16 | vertex colours << 17 | negative colour << 18
       float m_anchor = 0.f, slope = 0.f, s_anchor = 0.f, ds = 0.f;
       float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), dc = ca;
       int cq0 = 0, cq1 = 0, cq2 = 0, cq3 = 0;
@@ -544,7 +544,8 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
           if (a <= b) minor_window(g, min0, min0 + kTile - 1, a, b);
           if (a <= b) {
             const double u0 = ((double)maj0 + 0.5) - g.M0;
-            m_anchor = (float)(g.m0 + u0 * g.slope - (double)min0);
+            // minor coordinate of the column's sample, tile-relative, minus the pixel-centre 0.5
+            m_anchor = (float)(g.m0 + u0 * g.slope - (double)min0 - 0.5);
             slope = (float)g.slope;
             ca = col0[s];
             const float4 cb = col1 ? col1[s] : ca;
@@ -556,74 +557,113 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
             cq1 = fx(ca.y, cscale);
             cq2 = fx(ca.z, cscale);
             cq3 = fx(ca.w * ca.w, cscale);
-            packed = (int)(b - a + 1) | ((int)(a - maj0) << 8) | (g.xmajor << 16) | (vc << 17);
+            const int neg = (cq0 | cq1 | cq2 | cq3) < 0 ? 1 : 0;
+            packed = (int)(b - a + 1) | ((int)(a - maj0) << 8) | (g.xmajor << 16) | (vc << 17) | (neg << 18);
           }
         }
       }
       // ---- walk the pairs that have samples ----
+      // Pair j's samples are columns jq0 .. jq0 + jn - 1 of the tile (jn <= 64): lane t takes
+      // column jq0 + t. Deposits are 32x32 -> 64-bit products: (2^20 - wq) * c at minor jl and
+      // wq * c at jl + 1, which add up to exactly c << 20.
       uint64_t todo = __ballot(packed != 0);
       while (todo) {
         const int j = __builtin_ctzll(todo);
         todo &= todo - 1;
         const int jp = bcast_i(packed, j);
         const int jn = jp & 0xff, jq0 = (jp >> 8) & 0xff, jxm = (jp >> 16) & 1, jvc = (jp >> 17) & 1;
+        const int jneg = (jp >> 18) & 1;
         wave_samples += (unsigned long long)jn;
         const float jm = bcast_f(m_anchor, j), jsl = bcast_f(slope, j);
-        const long long c20_0 = (long long)bcast_i(cq0, j) << kWBits;
-        const long long c20_1 = (long long)bcast_i(cq1, j) << kWBits;
-        const long long c20_2 = (long long)bcast_i(cq2, j) << kWBits;
-        const long long c20_3 = (long long)bcast_i(cq3, j) << kWBits;
-        const int jcq0 = bcast_i(cq0, j), jcq1 = bcast_i(cq1, j), jcq2 = bcast_i(cq2, j),
-                  jcq3 = bcast_i(cq3, j);
-        for (int t = lane; t < jn; t += 64) {
-          const int q = jq0 + t;  // major index inside the tile
-          const float m = __builtin_fmaf((float)q, jsl, jm) - 0.5f;
-          const float jf = __builtin_floorf(m);
-          const float f = m - jf;
-          const int jl = (int)jf;  // local minor index of the first deposit pixel
-          int wq = (int)__builtin_fmaf(f, (float)kWOne, 0.5f);
-          wq = wq > kWOne ? kWOne : wq;
-          long long hi0, hi1, hi2, hi3, lo0, lo1, lo2, lo3;
-          if (!jvc) {
-            hi0 = (long long)wq * jcq0;
-            hi1 = (long long)wq * jcq1;
-            hi2 = (long long)wq * jcq2;
-            hi3 = (long long)wq * jcq3;
-            lo0 = c20_0 - hi0;
-            lo1 = c20_1 - hi1;
-            lo2 = c20_2 - hi2;
-            lo3 = c20_3 - hi3;
-          } else {
-            const float sp = __builtin_fmaf((float)q, bcast_f(ds, j), bcast_f(s_anchor, j));
-            const float al = __builtin_fmaf(sp, bcast_f(dc.w, j), bcast_f(ca.w, j));
-            const int c0 = fx(__builtin_fmaf(sp, bcast_f(dc.x, j), bcast_f(ca.x, j)), cscale);
-            const int c1 = fx(__builtin_fmaf(sp, bcast_f(dc.y, j), bcast_f(ca.y, j)), cscale);
-            const int c2 = fx(__builtin_fmaf(sp, bcast_f(dc.z, j), bcast_f(ca.z, j)), cscale);
+        // tile index of (major q, minor r) = r * rstep + q * qstep
+        const int rstep = jxm ? kTileStride : 1;
+        const int qstep = jxm ? 1 : kTileStride;
+        const int q = jq0 + lane;
+        if (!jvc && !jneg) {
+          // non-negative colour: unsigned 32x32 -> 64-bit products (one multiply per deposit)
+          const unsigned jc0 = (unsigned)bcast_i(cq0, j), jc1 = (unsigned)bcast_i(cq1, j),
+                         jc2 = (unsigned)bcast_i(cq2, j), jc3 = (unsigned)bcast_i(cq3, j);
+          if (lane < jn) {
+            const float m = __builtin_fmaf((float)q, jsl, jm);
+            const float jf = __builtin_floorf(m);
+            const int jl = (int)jf;
+            int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
+            wq = wq > kWOne ? kWOne : wq;
+            const unsigned uq = (unsigned)wq, ul = (unsigned)(kWOne - wq);
+            // 24-bit multiplies (jl >= -1 wherever a deposit is made); i1 = i0 + rstep
+            const int i1 = (int)(__umul24((unsigned)(jl + 1), (unsigned)rstep) + __umul24((unsigned)q, (unsigned)qstep));
+            const int i0 = i1 - rstep;
+            if ((unsigned)jl < (unsigned)kTile) {
+              atomicAdd(&s_tile[i0], (unsigned long long)ul * jc0);
+              atomicAdd(&s_tile[kPlane + i0], (unsigned long long)ul * jc1);
+              atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)ul * jc2);
+              atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)ul * jc3);
+            }
+            if ((unsigned)(jl + 1) < (unsigned)kTile) {
+              atomicAdd(&s_tile[i1], (unsigned long long)uq * jc0);
+              atomicAdd(&s_tile[kPlane + i1], (unsigned long long)uq * jc1);
+              atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)uq * jc2);
+              atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)uq * jc3);
+            }
+          }
+        } else if (!jvc) {
+          const int jc0 = bcast_i(cq0, j), jc1 = bcast_i(cq1, j), jc2 = bcast_i(cq2, j), jc3 = bcast_i(cq3, j);
+          if (lane < jn) {
+            const float m = __builtin_fmaf((float)q, jsl, jm);
+            const float jf = __builtin_floorf(m);
+            const int jl = (int)jf;  // tile-relative minor index of the first deposit pixel
+            int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
+            wq = wq > kWOne ? kWOne : wq;
+            const int wl = kWOne - wq;
+            // 24-bit multiplies (jl >= -1 wherever a deposit is made); i1 = i0 + rstep
+            const int i1 = (int)(__umul24((unsigned)(jl + 1), (unsigned)rstep) + __umul24((unsigned)q, (unsigned)qstep));
+            const int i0 = i1 - rstep;
+            if ((unsigned)jl < (unsigned)kTile) {
+              atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * jc0));
+              atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * jc1));
+              atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * jc2));
+              atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * jc3));
+            }
+            if ((unsigned)(jl + 1) < (unsigned)kTile) {
+              atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * jc0));
+              atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * jc1));
+              atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * jc2));
+              atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * jc3));
+            }
+          }
+        } else {
+          // per-vertex colours: the colour is interpolated along the segment at each sample
+          const float jds = bcast_f(ds, j), jsa = bcast_f(s_anchor, j);
+          const float a0 = bcast_f(ca.x, j), a1 = bcast_f(ca.y, j), a2 = bcast_f(ca.z, j), a3 = bcast_f(ca.w, j);
+          const float d0 = bcast_f(dc.x, j), d1 = bcast_f(dc.y, j), d2 = bcast_f(dc.z, j), d3 = bcast_f(dc.w, j);
+          if (lane < jn) {
+            const float m = __builtin_fmaf((float)q, jsl, jm);
+            const float jf = __builtin_floorf(m);
+            const int jl = (int)jf;
+            int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
+            wq = wq > kWOne ? kWOne : wq;
+            const int wl = kWOne - wq;
+            const float sp = __builtin_fmaf((float)q, jds, jsa);
+            const float al = __builtin_fmaf(sp, d3, a3);
+            const int c0 = fx(__builtin_fmaf(sp, d0, a0), cscale);
+            const int c1 = fx(__builtin_fmaf(sp, d1, a1), cscale);
+            const int c2 = fx(__builtin_fmaf(sp, d2, a2), cscale);
             const int c3 = fx(al * al, cscale);
-            hi0 = (long long)wq * c0;
-            hi1 = (long long)wq * c1;
-            hi2 = (long long)wq * c2;
-            hi3 = (long long)wq * c3;
-            lo0 = ((long long)c0 << kWBits) - hi0;
-            lo1 = ((long long)c1 << kWBits) - hi1;
-            lo2 = ((long long)c2 << kWBits) - hi2;
-            lo3 = ((long long)c3 << kWBits) - hi3;
-          }
-          // deposits: (1 - f) at minor jl, f at minor jl + 1
-          const int i0 = jxm ? (jl * kTileStride + q) : (q * kTileStride + jl);
-          const int step = jxm ? kTileStride : 1;
-          if (jl >= 0 && jl < kTile) {
-            atomicAdd(&s_tile[i0], (unsigned long long)lo0);
-            atomicAdd(&s_tile[kPlane + i0], (unsigned long long)lo1);
-            atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)lo2);
-            atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)lo3);
-          }
-          if (jl + 1 >= 0 && jl + 1 < kTile) {
-            const int i1 = i0 + step;
-            atomicAdd(&s_tile[i1], (unsigned long long)hi0);
-            atomicAdd(&s_tile[kPlane + i1], (unsigned long long)hi1);
-            atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)hi2);
-            atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)hi3);
+            // 24-bit multiplies (jl >= -1 wherever a deposit is made); i1 = i0 + rstep
+            const int i1 = (int)(__umul24((unsigned)(jl + 1), (unsigned)rstep) + __umul24((unsigned)q, (unsigned)qstep));
+            const int i0 = i1 - rstep;
+            if ((unsigned)jl < (unsigned)kTile) {
+              atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * c0));
+              atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * c1));
+              atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * c2));
+              atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * c3));
+            }
+            if ((unsigned)(jl + 1) < (unsigned)kTile) {
+              atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * c0));
+              atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * c1));
+              atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * c2));
+              atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * c3));
+            }
           }
         }
       }
