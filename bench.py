@@ -29,6 +29,8 @@ LDS_U64_ATOMIC_PEAK = 6.0e12
 # Brute-force algorithmic FP32 flops per traced segment of the large scene: 16 circles x 18,
 # 16 rects x 28, 16 lenses x 56, 16 mirrors x 12, plus shading 40 and canvas exit 12.
 FLOPS_PER_SEGMENT_LARGE = 16 * (18 + 28 + 56 + 12) + 52
+# the small scene: circle 18 + rect 28 + mirror 12 + cubic Bezier 120 + shading 40 + exit 12
+FLOPS_PER_SEGMENT_SMALL = 18 + 28 + 12 + 120 + 40 + 12
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "tile_accum_large_single_pmc.md")
 
 
@@ -349,53 +351,29 @@ def bench_serving(args):
     print(json.dumps(result))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rays", type=int, default=1_000_000)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline and image check)")
-    ap.add_argument("--cpu-stride", type=int, default=3, help="CPU baseline traces every n-th ray")
-    ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", default="large_single", choices=["large_single", "batched", "serving"])
-    ap.add_argument("--requests", type=int, default=1024, help="serving: requests per load level")
-    ap.add_argument("--hbm-budget-gib", type=int, default=64, help="serving: HBM budget")
-    ap.add_argument("--scenes", type=int, default=256, help="batched: number of scenes")
-    ap.add_argument("--autotune", action="store_true", help="batched: tune launch knobs first")
-    ap.add_argument("--no-tuning-cache", action="store_true", help="batched: ignore tuning_mi355x.json")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: every box here has one GPU; this build measures one GPU's share and "
-              "--gpus %d is not supported (sharding is a design note only)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    if args.copy_calibration:
-        copy_calibration()
-        return
-    if args.workload == "batched":
-        bench_batched(args)
-        return
-    if args.workload == "serving":
-        bench_serving(args)
-        return
-
+def bench_scene(args, scene, desc, tuning_key, flops_per_segment, cpu_stride):
+    """Whole frames (clear, trace, bin, tile accumulation, resolve) of one scene."""
     import light_garden_amd as lg
-    from light_garden_amd import abi, configs
-
-    if not os.path.exists(abi.LIB_PATH):
-        raise SystemExit("bench.py: %s is not built; run __graft_entry__.build()" % abi.LIB_PATH)
-
-    scene = configs.large_single(num_rays=args.rays)
+    from light_garden_amd import autotune
     ctx = lg.Context(0)
+    autotune.apply(ctx, tuning_key)
     tr = lg.Tracer(ctx, scene.canvas)
     for o in scene.objects:
         tr.push_object(o)
     for l in scene.lights:
         tr.push_light(l)
     tr.max_bounce = scene.max_bounce
+    tr.cutoff_color = list(scene.cutoff)
     segs = lg.Segments(ctx, capacity=8 * scene.n_rays())
     img = lg.Image(ctx, scene.width, scene.height, scene.canvas)
 
+    if args.autotune:
+        best, table = autotune.tune(ctx, lambda: tr.render(segs, img), repeats=5,
+                                    log=lambda m: print("autotune: " + m, file=sys.stderr, flush=True))
+        out = os.path.join(ROOT, "gpurun_out", "tuning_%s.json" % tuning_key)
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        autotune.save(tuning_key, best, min(r["ms"] for r in table), out,
+                      note="coordinate descent over %d settings" % len(table))
     for _ in range(args.warmup):
         tr.render(segs, img)
     ctx.synchronize()
@@ -445,29 +423,27 @@ def main():
     algo_bytes = 32.0 * n_seg + 32.0 * px
     t_ta = ta["avg_launch_ms"] * 1e-3
     achieved = algo_bytes / t_ta / 1e9
-    traffic = measured_traffic()
+    traffic = measured_traffic() if tuning_key == "large_single" else None
     roofline = {"bound": "hbm", "kernel": "tile_accum", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "algorithmic_bytes": algo_bytes, "avg_launch_ms": ta["avg_launch_ms"]}
     lds_rate = 8.0 * samples / t_ta
     trace_ms = sum(v["ms_per_frame"] for k, v in stages.items() if k.startswith("trace"))
-    trace_tflops = FLOPS_PER_SEGMENT_LARGE * n_seg / (trace_ms * 1e-3) / 1e12 if trace_ms else 0.0
+    trace_tflops = flops_per_segment * n_seg / (trace_ms * 1e-3) / 1e12 if trace_ms else 0.0
     secondary = {
         "tile_accum_lds_atomics": {"achieved": lds_rate, "peak": LDS_U64_ATOMIC_PEAK, "unit": "ds_add_u64/s",
                                    "frac": lds_rate / LDS_U64_ATOMIC_PEAK, "samples_per_frame": samples,
                                    "note": "8 LDS u64 atomics per anti-aliased sample (2 pixels x 4 channels)"},
         "trace_valu": {"achieved": trace_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": trace_tflops / FP32_PEAK_TFLOPS, "ms_per_frame": trace_ms,
-                       "note": "brute-force algorithmic flops, %d per segment" % FLOPS_PER_SEGMENT_LARGE},
+                       "note": "brute-force algorithmic flops, %d per segment" % flops_per_segment},
     }
 
     result = {
         "metric": METRIC, "value": value, "unit": "ray-segments/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "large_single: %d rays (point + 30deg spot + 0.4 directional, seed 2), "
-                               "64 objects (16 circle/rect/lens/mirror), 10 bounces, %dx%d RGBA16F"
-                               % (scene.n_rays(), scene.width, scene.height),
+        "config": {"workload": desc,
                    "segments_per_frame": n_seg, "trace": "fp32 culling + fp64 ray state and shading",
                    "tuning": ctx.tuning()},
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
@@ -478,8 +454,132 @@ def main():
         "roofline_secondary": secondary,
     }
     if not args.no_cpu:
-        cpu_leg(scene, img, result, max(1, args.cpu_stride))
+        cpu_leg(scene, img, result, max(1, cpu_stride))
     print(json.dumps(result))
+
+
+
+def bench_string_mod(args):
+    """The string-mod accumulation: 16 Mul-mode string mods (k = 2..17) of 100,000 points on the
+    circle of radius 0.9, 1.6M segments resident on the device, accumulated into 4096x4096 and
+    resolved to RGBA16F every frame."""
+    import numpy as np
+    import light_garden_amd as lg
+    from light_garden_amd import configs
+    spec = configs.StringModSpec()
+    pos, col = configs.string_mod_segments(spec)
+    ctx = lg.Context(0)
+    segs = lg.Segments(ctx, capacity=len(pos))
+    segs.upload(pos, col)
+    img = lg.Image(ctx, spec.width, spec.height)
+    n_seg = segs.count()
+
+    def frame():
+        img.clear()
+        img.accumulate(segs)
+        img.resolve()
+
+    for _ in range(args.warmup):
+        frame()
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    stage_sum, lat = {}, []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        img.clear()
+        add_stage_times(ctx, stage_sum)
+        img.accumulate(segs)
+        add_stage_times(ctx, stage_sum)
+        img.resolve()
+        add_stage_times(ctx, stage_sum)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    stages = stage_table(stage_sum, args.steps)
+    px = spec.width * spec.height
+    roof, lds = tile_accum_roofline(stages, n_seg, px)
+    result = {
+        "metric": METRIC, "value": n_seg * args.steps / elapsed, "unit": "ray-segments/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "string_mod: N=100000 points on a circle of radius 0.9, k=2..17 Mul mode "
+                               "(%d segments), palette seed 3, alpha 0.002, 4096x4096 RGBA16F, segments resident"
+                               % n_seg, "segments_per_frame": n_seg},
+        "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "stages": stages, "roofline": roof, "roofline_secondary": {"tile_accum_lds_atomics": lds},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        threads = cpu_threads()
+        ref = orc.accumulate_f64(pos, col, None, img.view, threads=threads)
+        result["rel_l1_vs_fp64"] = orc.rel_l1(img.rgba(), ref)
+        del ref
+        p32 = pos.astype(np.float32)
+        times = []
+        for rep in range(3):  # 1 warm-up + 2 timed, every segment
+            acc = np.zeros((spec.height, spec.width, 4), np.float32)
+            t1 = time.perf_counter()
+            orc.accumulate_f32(p32, col, None, img.view, threads=threads, acc=acc)
+            if rep:
+                times.append(time.perf_counter() - t1)
+        med = min(times)
+        result["cpu_baseline"] = {"value": n_seg / med, "unit": "ray-segments/s", "cores": threads, "kind": "port",
+                                  "sample": "all %d segments accumulated in FP32 by the CPU restatement "
+                                            "(row bands, %d threads), best of 2 after 1 warm-up, %.2f s"
+                                            % (n_seg, threads, med)}
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline and image check)")
+    ap.add_argument("--cpu-stride", type=int, default=3, help="CPU baseline traces every n-th ray")
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--workload", default="large_single",
+                    choices=["large_single", "plain_small", "string_mod", "batched", "serving"])
+    ap.add_argument("--requests", type=int, default=1024, help="serving: requests per load level")
+    ap.add_argument("--hbm-budget-gib", type=int, default=64, help="serving: HBM budget")
+    ap.add_argument("--scenes", type=int, default=256, help="batched: number of scenes")
+    ap.add_argument("--autotune", action="store_true", help="tune launch knobs first (scene and batched workloads)")
+    ap.add_argument("--no-tuning-cache", action="store_true", help="batched: ignore tuning_mi355x.json")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: every box here has one GPU; this build measures one GPU's share and "
+              "--gpus %d is not supported (sharding is a design note only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.workload == "batched":
+        bench_batched(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
+        return
+
+    from light_garden_amd import abi, configs
+    if not os.path.exists(abi.LIB_PATH):
+        raise SystemExit("bench.py: %s is not built; run __graft_entry__.build()" % abi.LIB_PATH)
+    if args.workload == "string_mod":
+        bench_string_mod(args)
+    elif args.workload == "plain_small":
+        scene = configs.plain_small()
+        bench_scene(args, scene, "plain_small: one white point light at the origin, 20000 rays; circle, rect, "
+                                 "flat mirror, 60-degree Bezier arc mirror; 5 bounces; 1024x1024 RGBA16F",
+                    "plain_small", FLOPS_PER_SEGMENT_SMALL, 1)
+    else:
+        scene = configs.large_single(num_rays=args.rays)
+        bench_scene(args, scene, "large_single: %d rays (point + 30deg spot + 0.4 directional, seed 2), 64 objects "
+                                 "(16 circle/rect/lens/mirror), 10 bounces, %dx%d RGBA16F"
+                                 % (scene.n_rays(), scene.width, scene.height),
+                    "large_single", FLOPS_PER_SEGMENT_LARGE, args.cpu_stride)
 
 
 if __name__ == "__main__":

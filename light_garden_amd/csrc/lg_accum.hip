@@ -242,8 +242,10 @@ __global__ __launch_bounds__(256) void bin_kernel(const SceneDesc* __restrict__ 
   __shared__ uint32_t s_chunk;
   const uint32_t n_chunks = ctl->n_chunks;
   const uint32_t chunk = ctl->chunk;
+  bool first = true;
   for (;;) {
-    if (threadIdx.x == 0) s_chunk = atomicAdd(&ctl->chunk_cursor[PASS], 1u);
+    if (threadIdx.x == 0) s_chunk = next_work_item(&ctl->chunk_cursor[PASS], n_chunks, first);
+    first = false;
     __syncthreads();
     const uint32_t ci = __builtin_amdgcn_readfirstlane(s_chunk);
     if (ci >= n_chunks) break;
@@ -506,8 +508,10 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
   if (ctl->flags & kCtlStop) return;  // a buffer overflowed: the host reruns the frame
   const uint32_t n_items = ctl->n_work;
   unsigned long long wave_samples = 0;  // wave-uniform
+  bool first = true;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&ctl->work_cursor, 1u);
+    if (threadIdx.x == 0) s_item = next_work_item(&ctl->work_cursor, n_items, first);
+    first = false;
     __syncthreads();
     const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= n_items) break;

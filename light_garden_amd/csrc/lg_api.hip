@@ -177,6 +177,21 @@ static int compile_prim(const lg_object& obj, int oi, const lg_shape& s, const F
         R.p[4 + c] = 3.0 * (p1 - p0);
         R.p[6 + c] = p0;
       }
+      // bounding box of the control points (the curve lies in their convex hull), padded so
+      // that the FP32 culling test never rejects a hit
+      double x0 = P[0], x1 = P[0], y0 = P[1], y1 = P[1], mag = 0.0;
+      for (int i = 0; i < 4; i++) {
+        x0 = std::min(x0, P[2 * i]);
+        x1 = std::max(x1, P[2 * i]);
+        y0 = std::min(y0, P[2 * i + 1]);
+        y1 = std::max(y1, P[2 * i + 1]);
+        mag = std::max(mag, std::max(std::fabs(P[2 * i]), std::fabs(P[2 * i + 1])));
+      }
+      const double pad = 1e-5 * (1.0 + mag);
+      R.p[8] = x0 - pad;
+      R.p[9] = y0 - pad;
+      R.p[10] = x1 + pad;
+      R.p[11] = y1 + pad;
       break;
     }
     case LG_SHAPE_CIRCLE: {

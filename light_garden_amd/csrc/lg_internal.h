@@ -235,6 +235,18 @@ struct WorkItem {
 };
 constexpr uint32_t kSplitBit = 0x80000000u;
 
+// Work distribution of the persistent kernels: workgroup b first takes item b; workgroups
+// that finish it take further items from `cursor`, which counts up from gridDim.x. A plain
+// (coherent) read of the cursor lets workgroups leave without touching the atomic once the
+// items are gone, so a launch that has fewer items than workgroups costs no atomics at all
+// beyond one per worked item.
+__device__ __forceinline__ uint32_t next_work_item(uint32_t* cursor, uint32_t n_items, bool first) {
+  if (first) return blockIdx.x;
+  if (__hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + gridDim.x >= n_items)
+    return n_items;
+  return atomicAdd(cursor, 1u) + gridDim.x;
+}
+
 // ---- error helpers (host) -------------------------------------------------------------
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);

@@ -17,7 +17,7 @@
 namespace lg {
 
 constexpr size_t kCounterStride = 128;  // per scene: u64 segments, then u32 per generation
-constexpr uint32_t kChunkMin = 1024;    // segments per bin chunk, at least
+constexpr uint32_t kChunkMin = 256;     // segments per bin chunk, at least
 constexpr uint32_t kBinWaves = 4;       // bin workgroups per compute unit (bin_target = 2 per workgroup)
 static_assert(8 + 4 * kMaxGen <= kCounterStride, "counter stride");
 // Pairs per tile work item: total / (2 * n_cu) + 1, clamped to [min_piece, max_piece]. A tile

@@ -245,6 +245,39 @@ __device__ __forceinline__ T horner3(T A, T B, T C, T D, T s) {
 // f(s) = n . (B(s) - o) into monotone pieces and each piece with a sign change is
 // bisected (the oracle's rule). Returns false on a miss; sets `ambiguous` (FP32 only)
 // when a sign test at a piece end is too close to zero to trust.
+// Whether the ray (t > T_MIN) can meet the padded box b = {x0, y0, x1, y1} (slab test).
+template <typename T>
+__device__ __forceinline__ bool ray_meets_box(const RayT<T>& r, const T* b) {
+  T lo = tmin_v<T>(), hi = inf_v<T>();
+  if (r.dx == (T)0) {
+    if (r.ox < b[0] || r.ox > b[2]) return false;
+  } else {
+    const T inv = (T)1 / r.dx;
+    T t0 = (b[0] - r.ox) * inv, t1 = (b[2] - r.ox) * inv;
+    if (t0 > t1) {
+      const T u = t0;
+      t0 = t1;
+      t1 = u;
+    }
+    lo = t0 > lo ? t0 : lo;
+    hi = t1 < hi ? t1 : hi;
+  }
+  if (r.dy == (T)0) {
+    if (r.oy < b[1] || r.oy > b[3]) return false;
+  } else {
+    const T inv = (T)1 / r.dy;
+    T t0 = (b[1] - r.oy) * inv, t1 = (b[3] - r.oy) * inv;
+    if (t0 > t1) {
+      const T u = t0;
+      t0 = t1;
+      t1 = u;
+    }
+    lo = t0 > lo ? t0 : lo;
+    hi = t1 < hi ? t1 : hi;
+  }
+  return lo <= hi;
+}
+
 template <typename T>
 __device__ __forceinline__ bool bezier_nearest(const RayT<T>& r, const T* a, T& t_best, T& s_best,
                                bool& ambiguous) {
@@ -316,23 +349,30 @@ __device__ __forceinline__ bool bezier_nearest(const RayT<T>& r, const T* a, T& 
     } else {
       if (fr == (T)0) continue;
       if ((fl < (T)0) == (fr < (T)0)) continue;
+      // safeguarded Newton on the bracket [lo, hi] (f(lo), f(hi) of opposite signs): a Newton
+      // step that leaves the bracket is replaced by its midpoint, so it converges like
+      // bisection in the worst case and quadratically near a simple root
       T lo = l, hi = rr, flo = fl;
+      T x = (T)0.5 * (lo + hi);
       for (int it = 0; it < kMaxIt; it++) {
-        T mid = (T)0.5 * (lo + hi);
-        if (mid <= lo || mid >= hi) break;
-        T fm = horner3(A, B, C, D, mid);
+        const T fm = horner3(A, B, C, D, x);
         if (fm == (T)0) {
-          lo = hi = mid;
+          lo = hi = x;
           break;
         }
         if ((fm < (T)0) == (flo < (T)0)) {
-          lo = mid;
+          lo = x;
           flo = fm;
         } else {
-          hi = mid;
+          hi = x;
         }
+        const T dfm = ((T)3 * A * x + (T)2 * B) * x + C;
+        T xn = x - fm / dfm;
+        if (!(xn > lo && xn < hi)) xn = (T)0.5 * (lo + hi);
+        if (xn <= lo || xn >= hi || xn == x) break;  // the bracket cannot shrink further
+        x = xn;
       }
-      root = (T)0.5 * (lo + hi);
+      root = lo == hi ? lo : x;
     }
     T px = ((a[0] * root + a[2]) * root + a[4]) * root + a[6];
     T py = ((a[1] * root + a[3]) * root + a[5]) * root + a[7];
@@ -477,6 +517,7 @@ __device__ __forceinline__ int eval_object(const RayT<T>& r, const Rec<T>* recs,
     return i + 1;
   }
   if (kind == REC_BEZ) {
+    if (!ray_meets_box(r, R.p + 8)) return i + 1;
     T t, s = (T)0;
     bool amb = false;
     bool hit = bezier_nearest(r, R.p, t, s, amb);
@@ -781,8 +822,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const SceneDesc* __restric
   const uint32_t n_items = ctl->n_items_gen[gen_ix];
   int loaded = -1;
 
+  bool first = true;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&ctl->item_cursor[gen_ix], 1u);
+    if (threadIdx.x == 0) s_item = next_work_item(&ctl->item_cursor[gen_ix], n_items, first);
+    first = false;
     __syncthreads();
     // the item (and so the scene) is the same for the whole workgroup: keep it scalar
     const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
