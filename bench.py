@@ -416,6 +416,13 @@ def bench_scene(args, scene, desc, tuning_key, flops_per_segment, cpu_stride):
                   "avg_launch_ms": v[0] / max(1, v[1]), "work_per_frame": v[2] / args.steps}
               for k, v in stage_sum.items()}
     px = scene.width * scene.height
+    # PCIe leg, not part of value: the RGBA16F image copied to the host after a frame
+    rb = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        img.rgba16f()
+        rb.append((time.perf_counter() - t1) * 1e3)
+    readback_ms = sorted(rb)[len(rb) // 2]
     ta = stages["tile_accum"]
     samples = ta["work_per_frame"]
     # algorithmic HBM bytes of one tile_accum launch: every segment read once (16 B end points +
@@ -449,6 +456,8 @@ def bench_scene(args, scene, desc, tuning_key, flops_per_segment, cpu_stride):
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
         "graph_replay": {"value": n_seg * args.steps / elapsed_g, "ms_per_step": elapsed_g * 1e3 / args.steps,
                          "latency_ms": {"p50": percentile(lat_g, 50), "p99": percentile(lat_g, 99)}},
+        "pcie_readback": {"ms": readback_ms, "bytes": px * 8,
+                          "value_incl_readback": n_seg / (ms_per_step * 1e-3 + readback_ms * 1e-3)},
         "stages": stages,
         "roofline": roofline,
         "roofline_secondary": secondary,

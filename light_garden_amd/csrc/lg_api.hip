@@ -795,9 +795,14 @@ int lg_image_destroy(lg_image* img) {
 
 int lg_image_clear(lg_image* img) {
   if (!img) return fail(LG_E_INVALID, "image is NULL");
+  lg_ctx* ctx = img->ctx;
+  LG_HIP(hipSetDevice(ctx->device));
+  begin_call(ctx);
   Entry e;
   e.img = img;
-  return pipeline_run(img->ctx, {e}, ST_CLEAR);
+  int rc = pipeline_run(ctx, {e}, ST_CLEAR);
+  end_call(ctx);
+  return rc;
 }
 
 int lg_image_device_ptrs(lg_image* img, void** accum, void** rgba16f) {
@@ -844,9 +849,12 @@ int lg_accumulate(lg_ctx* ctx, const lg_segments* segs, lg_image* img) {
 int lg_resolve(lg_ctx* ctx, lg_image* img) {
   if (!ctx || !img) return fail(LG_E_INVALID, "ctx or image is NULL");
   LG_HIP(hipSetDevice(ctx->device));
+  begin_call(ctx);
   Entry e;
   e.img = img;
-  return pipeline_run(ctx, {e}, ST_RESOLVE);
+  int rc = pipeline_run(ctx, {e}, ST_RESOLVE);
+  end_call(ctx);
+  return rc;
 }
 
 int lg_render(lg_scene* scene, const lg_params* params, lg_segments* segs, lg_image* img) {
