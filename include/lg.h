@@ -177,6 +177,29 @@ int lg_scene_remove_light(lg_scene* scene, uint32_t ix);
 int lg_scene_counts(const lg_scene* scene, uint32_t* n_objects, uint32_t* n_lights,
                     uint64_t* n_rays);
 
+/* ---- RON scene files: `(Vec<Object>, Vec<Light>)` as ron reads and writes it ----------
+ * Host only; no device work. The text is the reference's save format (default.ron; objects
+ * by object_enum variant, material_opt, moved; lights by variant). LineSegment is written
+ * `(a: [x, y], b: [x, y])` and ConvexPolygon `(points: [[x, y], ...])`, which the reference's
+ * files do not show. Errors: LG_E_INVALID for malformed text (the reference panics,
+ * tracer.rs:191-192), LG_E_CAPACITY with the needed count/length returned. */
+/* Tracer::load (tracer.rs:190-204): replaces all objects and lights; the scene is unchanged
+ * when the text does not parse. Light rays are regenerated from num_rays at trace time. */
+int lg_scene_load_ron(lg_scene* scene, const char* text, uint64_t len);
+/* Tracer::serialize (tracer.rs:183-188): ron::ser::to_string_pretty with PrettyConfig::default
+ * (4-space indent, trailing commas, floats in Rust's shortest Display form). `buf` gets the
+ * text and a NUL; *len is the text length without it. */
+int lg_scene_save_ron(const lg_scene* scene, char* buf, uint64_t cap, uint64_t* len);
+/* Context-free forms of the two above (LightGarden::load_from_file / save_to_file read and
+ * write the same text, mod.rs:698-725). Shape trees are decoded as lg_object nodes: Lens,
+ * Geo(GeoLogic) -> LOGIC root with children; a LOGIC root of two equal circles at (+-d, 0)
+ * under And is written back as Lens, other LOGIC roots as Geo(GeoLogic(...)). */
+int lg_ron_parse_scene(const char* text, uint64_t len, lg_object* objects, uint32_t cap_objects,
+                       uint32_t* n_objects, lg_light* lights, uint32_t cap_lights,
+                       uint32_t* n_lights);
+int lg_ron_write_scene(const lg_object* objects, uint32_t n_objects, const lg_light* lights,
+                       uint32_t n_lights, char* buf, uint64_t cap, uint64_t* len);
+
 /* ---- segments: the `Vec<(P2, Color)>` vertex pairs of trace_all (tracer.rs:451-486) --- */
 /* Device SoA: pos float4 {x0,y0,x1,y1}; color0 float4; optional color1 float4 (per-vertex
  * colours, string_mod.rs:118-119); key uint64 (ray index << 24 | path bits). */

@@ -170,6 +170,13 @@ FUNCTIONS = {
     "lg_server_submit": (C.c_int, [_P, C.POINTER(LgRequest), C.POINTER(C.c_uint64)]),
     "lg_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(LgResult)]),
     "lg_server_get_stats": (C.c_int, [_P, C.POINTER(LgServerStats)]),
+    "lg_scene_load_ron": (C.c_int, [_P, C.c_char_p, C.c_uint64]),
+    "lg_scene_save_ron": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "lg_ron_parse_scene": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(LgObject), C.c_uint32,
+                                     C.POINTER(C.c_uint32), C.POINTER(LgLight), C.c_uint32,
+                                     C.POINTER(C.c_uint32)]),
+    "lg_ron_write_scene": (C.c_int, [C.POINTER(LgObject), C.c_uint32, C.POINTER(LgLight),
+                                     C.c_uint32, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

@@ -248,3 +248,38 @@ def encode(geo, obj: abi.LgObject):
     for i, v in enumerate(verts):
         obj.vertices[i] = v
     obj.n_shapes = len(nodes)
+
+
+def decode(obj: abi.LgObject):
+    """The inverse of encode: obj's node tree (root at 0) as geometry values."""
+    n = int(obj.n_shapes)
+    if not 1 <= n <= abi.MAX_SHAPES:
+        raise ValueError("object has %d shape nodes" % n)
+
+    def node(i: int):
+        if not 0 <= i < n:
+            raise ValueError("shape node %d out of range" % i)
+        s = obj.shapes[i]
+        origin = (s.origin[0], s.origin[1])
+        rot = tuple(s.rotation[k] for k in range(4))
+        if s.kind == abi.SHAPE_LINE_SEGMENT:
+            return LineSegment((s.p[0], s.p[1]), (s.p[2], s.p[3]))
+        if s.kind == abi.SHAPE_CUBIC_BEZIER:
+            return CubicBezier([(s.p[2 * k], s.p[2 * k + 1]) for k in range(4)])
+        if s.kind == abi.SHAPE_CIRCLE:
+            return Circle(origin, s.p[0])
+        if s.kind == abi.SHAPE_RECT:
+            return Rect(origin, rot, s.p[0], s.p[1])
+        if s.kind == abi.SHAPE_ELLIPSE:
+            return Ellipse(origin, s.p[0], s.p[1], rot)
+        if s.kind == abi.SHAPE_CONVEX_POLYGON:
+            o = s.vertex_offset
+            verts = [(obj.vertices[2 * (o + k)], obj.vertices[2 * (o + k) + 1]) for k in range(s.n_vertices)]
+            return ConvexPolygon(verts, origin, rot)
+        if s.kind == abi.SHAPE_LOGIC:
+            if s.child_a <= i or s.child_b <= i:
+                raise ValueError("Logic node %d has children before it" % i)
+            return Logic(int(s.op), node(s.child_a), node(s.child_b), origin, rot)
+        raise ValueError("unknown shape kind %d" % s.kind)
+
+    return node(0)

@@ -79,6 +79,11 @@ class Object:
         self.geometry = self.geometry.translated(origin[0] - ox, origin[1] - oy)
         self.moved = True
 
+    @staticmethod
+    def from_abi(o: abi.LgObject) -> "Object":
+        mat = Material(float(o.refractive_index)) if o.has_material else None
+        return Object(G.decode(o), mat, False)
+
     def to_abi(self) -> abi.LgObject:
         o = abi.LgObject()
         G.encode(self.geometry, o)
@@ -145,6 +150,51 @@ class DirectionalLight:
 
     def get_origin(self):
         return tuple(self.start.a)
+
+
+def light_from_abi(L: abi.LgLight):
+    color = [float(L.color[k]) for k in range(4)]
+    pos = (L.position[0], L.position[1])
+    d = (L.direction[0], L.direction[1])
+    if L.kind == abi.LIGHT_POINT:
+        return PointLight(pos, int(L.num_rays), color)
+    if L.kind == abi.LIGHT_SPOT:
+        return SpotLight(pos, float(L.spot_angle), d, int(L.num_rays), color)
+    if L.kind == abi.LIGHT_DIRECTIONAL:
+        return DirectionalLight(color, int(L.num_rays), G.LineSegment(pos, d))
+    raise ValueError("unknown light kind %d" % L.kind)
+
+
+def parse_ron(text: str):
+    """RON scene text -> (objects, lights) (lg_ron_parse_scene; the reference's
+    ron::from_str::<(Vec<Object>, Vec<Light>)>, tracer.rs:191)."""
+    data = text.encode("utf-8")
+    no, nl = C.c_uint32(0), C.c_uint32(0)
+    lib = abi.load()
+    rc = lib.lg_ron_parse_scene(data, len(data), None, 0, C.byref(no), None, 0, C.byref(nl))
+    if rc not in (abi.LG_OK, abi.LG_E_CAPACITY):
+        abi.check(rc, "lg_ron_parse_scene")
+    objs = (abi.LgObject * max(1, no.value))()
+    lights = (abi.LgLight * max(1, nl.value))()
+    abi.call("lg_ron_parse_scene", data, len(data), objs, no.value, C.byref(no), lights, nl.value,
+             C.byref(nl))
+    return ([Object.from_abi(objs[i]) for i in range(no.value)],
+            [light_from_abi(lights[i]) for i in range(nl.value)])
+
+
+def write_ron(objects, lights) -> str:
+    """(objects, lights) -> RON text in ron's pretty form (lg_ron_write_scene; the
+    reference's Tracer::serialize, tracer.rs:183-188)."""
+    objs = objects_array(objects)
+    ls = lights_array(lights)
+    n = C.c_uint64(0)
+    lib = abi.load()
+    rc = lib.lg_ron_write_scene(objs, len(objects), ls, len(lights), None, 0, C.byref(n))
+    if rc not in (abi.LG_OK, abi.LG_E_CAPACITY):
+        abi.check(rc, "lg_ron_write_scene")
+    buf = C.create_string_buffer(n.value + 1)
+    abi.call("lg_ron_write_scene", objs, len(objects), ls, len(lights), buf, n.value + 1, C.byref(n))
+    return buf.value.decode("utf-8")
 
 
 def objects_array(objs: Sequence[Object]):
@@ -397,6 +447,19 @@ class Tracer:
 
     def clear_objects(self):
         self.objects.clear()
+        self._synced = False
+
+    def serialize(self) -> str:
+        """Tracer::serialize (tracer.rs:183-188): the scene as RON text."""
+        return write_ron(self.objects, self.lights)
+
+    def load(self, data: str):
+        """Tracer::load (tracer.rs:190-204): replaces objects and lights with the RON
+        text's. Raises abi.LgError (and keeps the scene) when the text does not parse."""
+        objects, lights = parse_ron(data)
+        self.clear()
+        self.objects = objects
+        self.lights = lights
         self._synced = False
 
     def object_iterator(self):
