@@ -125,9 +125,9 @@ def add_stage_times(ctx, stage_sum):
         s[2] += work
 
 
-def tile_accum_roofline(stages, n_seg, px):
+def tile_accum_roofline(stages, n_seg, px, seg_bytes=32.0):
     ta = stages["tile_accum"]
-    algo_bytes = 32.0 * n_seg + 32.0 * px
+    algo_bytes = seg_bytes * n_seg + 32.0 * px
     t_ta = ta["avg_launch_ms"] * 1e-3
     achieved = algo_bytes / t_ta / 1e9
     lds_rate = 8.0 * ta["work_per_frame"] / t_ta
@@ -469,21 +469,23 @@ def bench_scene(args, scene, desc, tuning_key, flops_per_segment, cpu_stride):
 
 
 def bench_string_mod(args):
-    """The string-mod accumulation: 16 Mul-mode string mods (k = 2..17) of 100,000 points on the
-    circle of radius 0.9, 1.6M segments resident on the device, accumulated into 4096x4096 and
-    resolved to RGBA16F every frame."""
+    """The string-mod frame: 16 Mul-mode StringMods (k = 2..17) of 100,000 points on the circle
+    of radius 0.9 generated on the GPU (lg_string_mod_draw, 1.6M segments), accumulated into
+    4096x4096 and resolved to RGBA16F every frame (LightGarden::draw in Mode::StringMod, then
+    the LineList draw)."""
     import numpy as np
     import light_garden_amd as lg
     from light_garden_amd import configs
+    from light_garden_amd import string_mod as S
     spec = configs.StringModSpec()
-    pos, col = configs.string_mod_segments(spec)
+    mods = [S.StringMod(modulo=spec.modulo, num=k, color=c, scale=spec.scale)
+            for k, c in zip(spec.mults, spec.colors())]
     ctx = lg.Context(0)
-    segs = lg.Segments(ctx, capacity=len(pos))
-    segs.upload(pos, col)
+    segs = lg.Segments(ctx, capacity=len(mods) * spec.modulo, vertex_colors=True)
     img = lg.Image(ctx, spec.width, spec.height)
-    n_seg = segs.count()
 
     def frame():
+        S.draw(ctx, mods, segs)
         img.clear()
         img.accumulate(segs)
         img.resolve()
@@ -491,11 +493,14 @@ def bench_string_mod(args):
     for _ in range(args.warmup):
         frame()
     ctx.synchronize()
+    n_seg = segs.count()
     ctx.set_profiling(True)
     stage_sum, lat = {}, []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         t1 = time.perf_counter()
+        S.draw(ctx, mods, segs)
+        add_stage_times(ctx, stage_sum)
         img.clear()
         add_stage_times(ctx, stage_sum)
         img.accumulate(segs)
@@ -508,36 +513,38 @@ def bench_string_mod(args):
     ctx.set_profiling(False)
     stages = stage_table(stage_sum, args.steps)
     px = spec.width * spec.height
-    roof, lds = tile_accum_roofline(stages, n_seg, px)
+    roof, lds = tile_accum_roofline(stages, n_seg, px, seg_bytes=48.0)  # pos + two vertex colours
     result = {
         "metric": METRIC, "value": n_seg * args.steps / elapsed, "unit": "ray-segments/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "string_mod: N=100000 points on a circle of radius 0.9, k=2..17 Mul mode "
-                               "(%d segments), palette seed 3, alpha 0.002, 4096x4096 RGBA16F, segments resident"
-                               % n_seg, "segments_per_frame": n_seg},
+                               "(%d segments) generated on the GPU every frame, palette seed 3, alpha 0.002, "
+                               "4096x4096 RGBA16F" % n_seg, "segments_per_frame": n_seg},
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
         "stages": stages, "roofline": roof, "roofline_secondary": {"tile_accum_lds_atomics": lds},
     }
     if not args.no_cpu:
         from oracle import oracle as orc
         threads = cpu_threads()
+        pos, col = configs.string_mod_segments(spec)
         ref = orc.accumulate_f64(pos, col, None, img.view, threads=threads)
         result["rel_l1_vs_fp64"] = orc.rel_l1(img.rgba(), ref)
         del ref
-        p32 = pos.astype(np.float32)
         times = []
         for rep in range(3):  # 1 warm-up + 2 timed, every segment
             acc = np.zeros((spec.height, spec.width, 4), np.float32)
             t1 = time.perf_counter()
+            cpos = [orc.string_mod(spec.modulo, k, 1, 1, spec.scale, c)[0] for k, c in zip(spec.mults, spec.colors())]
+            p32 = np.concatenate(cpos).astype(np.float32)
             orc.accumulate_f32(p32, col, None, img.view, threads=threads, acc=acc)
             if rep:
                 times.append(time.perf_counter() - t1)
         med = min(times)
         result["cpu_baseline"] = {"value": n_seg / med, "unit": "ray-segments/s", "cores": threads, "kind": "port",
-                                  "sample": "all %d segments accumulated in FP32 by the CPU restatement "
-                                            "(row bands, %d threads), best of 2 after 1 warm-up, %.2f s"
-                                            % (n_seg, threads, med)}
+                                  "sample": "all %d segments generated (one thread) and accumulated in FP32 "
+                                            "(row bands, %d threads) by the CPU restatement, best of 2 after "
+                                            "1 warm-up, %.2f s" % (n_seg, threads, med)}
         result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
     print(json.dumps(result))
 

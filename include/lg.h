@@ -252,6 +252,50 @@ int lg_resolve(lg_ctx* ctx, lg_image* img);
  * mod.rs:680-696, renderer.rs:420). `segs` receives the trace output. */
 int lg_render(lg_scene* scene, const lg_params* params, lg_segments* segs, lg_image* img);
 
+/* ---- string mod: StringMod::draw on the GPU (string_mod.rs:3-158) ---------------------- */
+enum lg_sm_mode { LG_SM_ADD = 0, LG_SM_MUL = 1, LG_SM_POW = 2, LG_SM_BASE = 3 };  /* StringModMode */
+enum lg_sm_curve {                                                    /* Curve (string_mod.rs:182-188) */
+  LG_CURVE_CIRCLE = 0,        /* unit circle                                                  */
+  LG_CURVE_COMPLEX_EXP = 1,   /* c^(turns n): curve_f = {c.re, c.im}                          */
+  LG_CURVE_HYPOTROCHOID = 2,  /* curve_u = {r, s, d}                                          */
+  LG_CURVE_LISSAJOUS = 3      /* curve_u = {a, b}, curve_f[0] = delta                         */
+};
+#define LG_MAX_MOD_COLORS 256           /* modulo colours per StringMod                         */
+#define LG_MAX_CROSSING_LINES (1u << 18) /* lines of a nested StringMod's outer (O(L^2) pairs)   */
+typedef struct lg_mod_rem_color {       /* ModRemColor (string_mod.rs:166-171)                  */
+  uint64_t modulo;                      /* > 0                                                   */
+  uint64_t rem;
+  float color[4];
+} lg_mod_rem_color;
+typedef struct lg_string_mod {          /* StringMod (string_mod.rs:3-15)                        */
+  uint64_t modulo;
+  uint64_t num;
+  uint64_t turns;
+  uint32_t mode;                        /* lg_sm_mode                                            */
+  uint32_t curve;                       /* lg_sm_curve                                           */
+  uint64_t curve_u[3];
+  double curve_f[2];
+  double scale;                         /* curve radius; 1 = the reference's curves              */
+  float color[4];
+  const lg_mod_rem_color* modulo_colors;
+  uint32_t n_modulo_colors;
+  uint32_t _pad0;
+  const struct lg_string_mod* nested;   /* Option<Box<StringMod>>; NULL = None. Its own nested
+                                           is not read (draw() nests one level)             */
+} lg_string_mod;
+/* LightGarden::draw in Mode::StringMod (mod.rs:681-689): the segments of every StringMod's
+ * draw() (string_mod.rs:152-158), concatenated in order, into `out` (device; created with
+ * LG_SEG_VERTEX_COLORS; replaces its contents). A nested StringMod draws over the crossing
+ * points of its outer lines. Points in FP64, stored as FP32 vertices; key = emission index.
+ * LG_E_INVALID for an unknown mode/curve, a modulo colour with modulo 0 (the reference divides
+ * by it), more than LG_MAX_MOD_COLORS colours or a nested outer above LG_MAX_CROSSING_LINES. */
+int lg_string_mod_draw(lg_ctx* ctx, uint32_t n, const lg_string_mod* mods, lg_segments* out);
+/* StringMod::line_crossings_as_points (string_mod.rs:87-102): the crossing points of sm's
+ * lines, every ordered pair in the reference's order, 2 doubles each into xy. LG_E_CAPACITY
+ * with *n_points set when cap_points is too small. */
+int lg_string_mod_crossings(lg_ctx* ctx, const lg_string_mod* sm, double* xy, uint64_t cap_points,
+                            uint64_t* n_points);
+
 /* ---- batched form (no reference counterpart: the reference renders one scene per frame) --
  * lg_render for n scenes in one frame: every stage runs once over all scenes (persistent
  * work-queue kernels over device-built work lists) and the launch sequence is replayed from a

@@ -112,6 +112,28 @@ class LgServerStats(C.Structure):
                 ("busy_us", C.c_double)]
 
 
+SM_ADD, SM_MUL, SM_POW, SM_BASE = 0, 1, 2, 3
+CURVE_CIRCLE, CURVE_COMPLEX_EXP, CURVE_HYPOTROCHOID, CURVE_LISSAJOUS = 0, 1, 2, 3
+MAX_MOD_COLORS = 256
+MAX_CROSSING_LINES = 1 << 18
+
+
+class LgModRemColor(C.Structure):
+    _fields_ = [("modulo", C.c_uint64), ("rem", C.c_uint64), ("color", C.c_float * 4)]
+
+
+class LgStringMod(C.Structure):
+    pass
+
+
+LgStringMod._fields_ = [
+    ("modulo", C.c_uint64), ("num", C.c_uint64), ("turns", C.c_uint64), ("mode", C.c_uint32),
+    ("curve", C.c_uint32), ("curve_u", C.c_uint64 * 3), ("curve_f", C.c_double * 2), ("scale", C.c_double),
+    ("color", C.c_float * 4), ("modulo_colors", C.POINTER(LgModRemColor)), ("n_modulo_colors", C.c_uint32),
+    ("_pad0", C.c_uint32), ("nested", C.POINTER(LgStringMod)),
+]
+
+
 assert C.sizeof(LgShape) == 136
 assert C.sizeof(LgObject) == 16 + 7 * 136 + 256
 assert C.sizeof(LgLight) == 72
@@ -170,6 +192,9 @@ FUNCTIONS = {
     "lg_server_submit": (C.c_int, [_P, C.POINTER(LgRequest), C.POINTER(C.c_uint64)]),
     "lg_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(LgResult)]),
     "lg_server_get_stats": (C.c_int, [_P, C.POINTER(LgServerStats)]),
+    "lg_string_mod_draw": (C.c_int, [_P, C.c_uint32, C.POINTER(LgStringMod), _P]),
+    "lg_string_mod_crossings": (C.c_int, [_P, C.POINTER(LgStringMod), _P, C.c_uint64,
+                                          C.POINTER(C.c_uint64)]),
     "lg_scene_load_ron": (C.c_int, [_P, C.c_char_p, C.c_uint64]),
     "lg_scene_save_ron": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "lg_ron_parse_scene": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(LgObject), C.c_uint32,

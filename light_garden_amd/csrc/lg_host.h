@@ -62,6 +62,10 @@ struct Pipeline;
 struct PipelineDeleter {
   void operator()(Pipeline* p) const;
 };
+struct SmScratch;  // string-mod generator buffers (lg_stringmod.hip)
+struct SmScratchDeleter {
+  void operator()(SmScratch* p) const;
+};
 
 int fail(int code, const std::string& msg);
 
@@ -119,6 +123,7 @@ struct lg_ctx {
   std::vector<hipEvent_t> event_pool;
   std::vector<lg_stage_time> last_times;
   std::unique_ptr<lg::Pipeline, lg::PipelineDeleter> pipe;
+  std::unique_ptr<lg::SmScratch, lg::SmScratchDeleter> sm;
 
   hipEvent_t get_event() {
     if (!event_pool.empty()) {

@@ -1,0 +1,498 @@
+// lg_stringmod.hip — StringMod::draw on the GPU (src/light_garden/string_mod.rs:3-158): the
+// string-art producer that feeds the same accumulation as the trace.
+//
+//   curve points   init_points (string_mod.rs:33-85): Circle, ComplexExp, Hypotrochoid,
+//                  Lissajous, each point computed on the fly from its index in FP64
+//   segments       draw_init_points (104-122): segment iix -> ix(iix) for iix in 0..modulo,
+//                  per-vertex colours from get_color (124-150), FP64 end points stored as
+//                  FP32 like the reference's vertex upload (sub_render_pass.rs:188-196)
+//   nested         draw (152-158) with `nested`: the nested StringMod draws over the crossing
+//                  points of the outer one's lines, line_crossings_as_points (87-102): every
+//                  ordered pair (ixa, (ixa + diff) % L), diff = 1..L-1 outer, ixa inner, in
+//                  that order
+//
+// All FP64 arithmetic here is written without fused multiply-adds (contract off), so every
+// value is the IEEE result of the same operations in the same order as the reference's Rust
+// (which never fuses): crossings and Mul/Add/Pow/Base indices are bit-exact, ComplexExp points
+// too; the trigonometric curves differ from the host's libm by at most an ulp or two.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lg_host.h"
+
+#pragma clang fp contract(off)
+
+using lg::fail;
+
+namespace lg {
+
+struct SmDev {          // one StringMod as the generator reads it
+  uint64_t modulo, num, turns;
+  uint32_t mode, curve;
+  uint64_t cu[3];
+  double cf[2];
+  double scale;
+  float color[4];
+  uint32_t mrc_off, n_mrc;
+  uint64_t seg_base;    // first output segment of this StringMod
+  uint64_t n_points;    // points indexed: modulo (own curve) or the crossing count (nested)
+  const double2* pts;   // NULL: the curve's own points; else the crossing points
+};
+
+struct SmScratch {
+  DevBuf descs, mrc, lines, row_counts, row_off, total, maxbits;
+  std::vector<DevBuf*> crossings;  // crossing points of nested StringMods, per call
+  PinnedBuf stage;
+  ~SmScratch() {
+    for (DevBuf* b : crossings) delete b;
+  }
+};
+
+void SmScratchDeleter::operator()(SmScratch* s) const { delete s; }
+
+}  // namespace lg
+
+namespace {
+
+using lg::SmDev;
+
+constexpr int kBlock = 256;
+constexpr double kTau = 6.283185307179586;  // std::f64::consts::TAU
+
+__host__ __device__ inline uint64_t wrapping_pow(uint64_t base, uint32_t exp) {
+  // u64::pow; a release build wraps modulo 2^64, where every multiplication order agrees
+  uint64_t acc = 1;
+  while (exp) {
+    if (exp & 1u) acc *= base;
+    base *= base;
+    exp >>= 1;
+  }
+  return acc;
+}
+
+// ix for iix (string_mod.rs:111-117)
+__device__ inline uint64_t target_index(const SmDev& s, uint64_t iix) {
+  const uint64_t m = s.modulo;
+  switch (s.mode) {
+    case LG_SM_ADD: return (iix + s.num) % m;
+    case LG_SM_MUL: return (iix * s.num) % m;
+    case LG_SM_POW: return wrapping_pow(iix, (uint32_t)s.num) % m;
+    default: return wrapping_pow(s.num, (uint32_t)iix) % m;
+  }
+}
+
+// Complex::powu (num-complex 0.4.6): square-and-multiply over Complex * Complex =
+// (re1 re2 - im1 im2) + (re1 im2 + im1 re2) i.
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cpowu(double2 base, uint32_t exp) {
+  if (exp == 0) return make_double2(1.0, 0.0);
+  while ((exp & 1u) == 0) {
+    base = cmul(base, base);
+    exp >>= 1;
+  }
+  if (exp == 1) return base;
+  double2 acc = base;
+  while (exp > 1) {
+    exp >>= 1;
+    base = cmul(base, base);
+    if (exp & 1u) acc = cmul(acc, base);
+  }
+  return acc;
+}
+
+// init_points()[n] (string_mod.rs:35-83), times this build's radius `scale` (exact for 1).
+__device__ inline double2 curve_point(const SmDev& s, uint64_t n) {
+  const double angle = (double)(s.turns * n) * kTau / (double)s.modulo;
+  double x, y;
+  switch (s.curve) {
+    case LG_CURVE_COMPLEX_EXP: {
+      const double2 c = cpowu(make_double2(s.cf[0], s.cf[1]), (uint32_t)(s.turns * n));
+      x = c.x;
+      y = c.y;
+      break;
+    }
+    case LG_CURVE_HYPOTROCHOID: {
+      const double small_r = (double)s.cu[0], big_r = (double)s.cu[1], off = (double)s.cu[2];
+      const double smr = big_r - small_r;
+      const double ratio = smr + off;
+      const double inner = angle * smr / small_r;
+      x = (smr * cos(angle) + off * cos(inner)) / ratio;
+      y = (smr * sin(angle) - off * sin(inner)) / ratio;
+      break;
+    }
+    case LG_CURVE_LISSAJOUS:
+      x = sin((double)s.cu[0] * angle + s.cf[0]);
+      y = sin((double)s.cu[1] * angle);
+      break;
+    default:  // Circle: (y, x) = angle.sin_cos()
+      sincos(angle, &y, &x);
+      break;
+  }
+  return make_double2(x * s.scale, y * s.scale);
+}
+
+// get_color (string_mod.rs:124-150): the mean of the matching modulo colours, else the colour.
+__device__ inline float4 mod_color(const SmDev& s, const lg_mod_rem_color* mrc, uint64_t ix) {
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < s.n_mrc; i++) {
+    const lg_mod_rem_color& c = mrc[s.mrc_off + i];
+    if (ix % c.modulo == c.rem) {
+      acc[0] += c.color[0];
+      acc[1] += c.color[1];
+      acc[2] += c.color[2];
+      acc[3] += c.color[3];
+      k++;
+    }
+  }
+  if (k == 0) return make_float4(s.color[0], s.color[1], s.color[2], s.color[3]);
+  const float d = (float)k;
+  return make_float4(acc[0] / d, acc[1] / d, acc[2] / d, acc[3] / d);
+}
+
+__device__ inline float absmax4(float4 c) {
+  return fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(fabsf(c.z), fabsf(c.w)));
+}
+
+// One thread per output segment of every StringMod of the call.
+__global__ void __launch_bounds__(kBlock) sm_segments_kernel(const SmDev* __restrict__ descs, int n_descs,
+                                                             const lg_mod_rem_color* __restrict__ mrc,
+                                                             uint64_t n_segs, float4* __restrict__ pos,
+                                                             float4* __restrict__ col0, float4* __restrict__ col1,
+                                                             uint64_t* __restrict__ key,
+                                                             unsigned int* __restrict__ maxbits) {
+  const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  float mx = 0.0f;
+  if (g < n_segs) {
+    int lo = 0, hi = n_descs - 1;  // last descriptor with seg_base <= g
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].seg_base <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const SmDev& s = descs[lo];
+    const uint64_t iix = g - s.seg_base;
+    const uint64_t ix = target_index(s, iix);
+    double2 p, q;
+    if (s.pts) {
+      p = s.pts[iix % s.n_points];
+      q = s.pts[ix % s.n_points];
+    } else {
+      p = curve_point(s, iix);  // points[iix % m] with iix < m
+      q = curve_point(s, ix);
+    }
+    const float4 c0 = mod_color(s, mrc, iix);
+    const float4 c1 = mod_color(s, mrc, ix);
+    pos[g] = make_float4((float)p.x, (float)p.y, (float)q.x, (float)q.y);
+    col0[g] = c0;
+    col1[g] = c1;
+    key[g] = g;
+    mx = fmaxf(absmax4(c0), absmax4(c1));
+  }
+  // |colour| bits order like the floats; one atomic per wave
+  unsigned int b = __float_as_uint(mx);
+  for (int off = 32; off > 0; off >>= 1) b = max(b, (unsigned int)__shfl_xor((int)b, off, 64));
+  if ((threadIdx.x & 63) == 0 && b) atomicMax(maxbits, b);
+}
+
+// The outer StringMod's lines: a = points[ixa], r = points[ix(ixa)] - a.
+__global__ void __launch_bounds__(kBlock) sm_lines_kernel(SmDev s, double4* __restrict__ lines) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= s.modulo) return;
+  const double2 a = curve_point(s, i);
+  const double2 b = curve_point(s, target_index(s, i));
+  lines[i] = make_double4(a.x, a.y, b.x - a.x, b.y - a.y);
+}
+
+// LineSegment::intersect, restated (collision2d is absent): a + t r = c + u s with
+// t = ((c - a) x s) / (r x s), u = ((c - a) x r) / (r x s); a point iff r x s != 0 and
+// t, u in [0, 1]; the point is a + t r. Parallel and zero-length lines never meet.
+__device__ inline bool crossing(const double4 A, const double4 B, double2* out) {
+  const double denom = A.z * B.w - A.w * B.z;
+  if (denom == 0.0) return false;
+  const double qx = B.x - A.x, qy = B.y - A.y;
+  const double t = (qx * B.w - qy * B.z) / denom;
+  const double u = (qx * A.w - qy * A.z) / denom;
+  if (!(t >= 0.0 && t <= 1.0 && u >= 0.0 && u <= 1.0)) return false;
+  *out = make_double2(A.x + t * A.z, A.y + t * A.w);
+  return true;
+}
+
+// Block-wide exclusive prefix of a per-thread flag; returns the block total.
+__device__ inline uint32_t block_prefix(bool f, uint32_t* lds, uint32_t* prefix) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long bal = __ballot(f);
+  const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  if (lane == 0) lds[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (int w = 0; w < kBlock / 64; w++) {
+    const uint32_t c = lds[w];
+    before += w < wave ? c : 0u;
+    total += c;
+  }
+  __syncthreads();
+  *prefix = before + below;
+  return total;
+}
+
+// Row `diff` = blockIdx.x + 1 of the pair sweep. PASS 0 counts the row's crossings into
+// row_counts; PASS 1 writes them at row_off[row] in ixa order.
+template <int PASS>
+__global__ void __launch_bounds__(kBlock) sm_crossings_kernel(const double4* __restrict__ lines, uint32_t L,
+                                                              uint32_t* __restrict__ row_counts,
+                                                              const uint64_t* __restrict__ row_off,
+                                                              double2* __restrict__ out) {
+  __shared__ uint32_t lds[kBlock / 64];
+  const uint32_t diff = blockIdx.x + 1;
+  uint64_t base = PASS ? row_off[blockIdx.x] : 0;
+  uint32_t count = 0;
+  for (uint32_t c0 = 0; c0 < L; c0 += kBlock) {
+    const uint32_t ixa = c0 + threadIdx.x;
+    bool hit = false;
+    double2 p;
+    if (ixa < L) {
+      uint32_t j = ixa + diff;
+      if (j >= L) j -= L;
+      hit = crossing(lines[ixa], lines[j], &p);
+    }
+    if (PASS == 0) {
+      count += __popcll(__ballot(hit)) * (uint32_t)((threadIdx.x & 63) == 0);
+    } else {
+      uint32_t pre;
+      const uint32_t tot = block_prefix(hit, lds, &pre);
+      if (hit) out[base + pre] = p;
+      base += tot;
+    }
+  }
+  if (PASS == 0) {
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < kBlock / 64; w++) t += lds[w];
+      row_counts[blockIdx.x] = t;
+    }
+  }
+}
+
+// Exclusive prefix of n row counts into u64 offsets and their total (one block).
+__global__ void __launch_bounds__(1024) sm_scan_rows_kernel(const uint32_t* __restrict__ in, uint64_t n,
+                                                            uint64_t* __restrict__ out,
+                                                            uint64_t* __restrict__ total) {
+  __shared__ uint64_t part[1024];
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
+  uint64_t s = 0;
+  for (uint64_t i = b; i < e; i++) s += in[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint64_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t run = part[threadIdx.x] - s;
+  for (uint64_t i = b; i < e; i++) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (threadIdx.x == 1023) *total = part[1023];
+}
+
+__global__ void sm_zero_u32_kernel(unsigned int* p) { *p = 0u; }
+
+int check_mod(const lg_string_mod& m, bool nested) {
+  const char* who = nested ? "nested string mod" : "string mod";
+  if (m.mode > LG_SM_BASE) return fail(LG_E_INVALID, std::string(who) + ": unknown mode");
+  if (m.curve > LG_CURVE_LISSAJOUS) return fail(LG_E_INVALID, std::string(who) + ": unknown curve");
+  if (m.n_modulo_colors > LG_MAX_MOD_COLORS)
+    return fail(LG_E_INVALID, std::string(who) + ": more than LG_MAX_MOD_COLORS modulo colours");
+  if (m.n_modulo_colors && !m.modulo_colors) return fail(LG_E_INVALID, std::string(who) + ": modulo_colors is NULL");
+  for (uint32_t i = 0; i < m.n_modulo_colors; i++)
+    if (m.modulo_colors[i].modulo == 0)
+      return fail(LG_E_INVALID, std::string(who) + ": a modulo colour has modulo 0 (the reference divides by it)");
+  return LG_OK;
+}
+
+SmDev make_dev(const lg_string_mod& m, uint32_t mrc_off) {
+  SmDev d{};
+  d.modulo = m.modulo;
+  d.num = m.num;
+  d.turns = m.turns;
+  d.mode = m.mode;
+  d.curve = m.curve;
+  for (int i = 0; i < 3; i++) d.cu[i] = m.curve_u[i];
+  d.cf[0] = m.curve_f[0];
+  d.cf[1] = m.curve_f[1];
+  d.scale = m.scale;
+  for (int c = 0; c < 4; c++) d.color[c] = m.color[c];
+  d.mrc_off = mrc_off;
+  d.n_mrc = m.n_modulo_colors;
+  d.n_points = m.modulo;
+  return d;
+}
+
+lg::SmScratch& scratch(lg_ctx* ctx) {
+  if (!ctx->sm) ctx->sm.reset(new lg::SmScratch());
+  return *ctx->sm;
+}
+
+// Crossing points of `outer`'s lines into `dst` (device); *count gets their number.
+int crossings_device(lg_ctx* ctx, const lg_string_mod& outer, lg::DevBuf& dst, uint64_t* count) {
+  lg::SmScratch& S = scratch(ctx);
+  hipStream_t st = ctx->stream;
+  *count = 0;
+  const uint64_t L = outer.modulo;
+  if (L > LG_MAX_CROSSING_LINES)
+    return fail(LG_E_INVALID, "string mod: line_crossings_as_points over more than LG_MAX_CROSSING_LINES lines");
+  if (L < 2) return LG_OK;
+  int rc;
+  if ((rc = S.lines.ensure(L * sizeof(double4))) || (rc = S.row_counts.ensure((L - 1) * sizeof(uint32_t))) ||
+      (rc = S.row_off.ensure((L - 1) * sizeof(uint64_t))) || (rc = S.total.ensure(sizeof(uint64_t))))
+    return rc;
+  const SmDev d = make_dev(outer, 0);
+  {
+    lg::StageScope sc(ctx, "sm_lines", L);
+    hipLaunchKernelGGL(sm_lines_kernel, dim3((unsigned)((L + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d,
+                       S.lines.as<double4>());
+  }
+  {
+    lg::StageScope sc(ctx, "sm_cross_count", L * (L - 1));
+    hipLaunchKernelGGL(sm_crossings_kernel<0>, dim3((unsigned)(L - 1)), dim3(kBlock), 0, st,
+                       S.lines.as<const double4>(), (uint32_t)L, S.row_counts.as<uint32_t>(),
+                       (const uint64_t*)nullptr, (double2*)nullptr);
+  }
+  hipLaunchKernelGGL(sm_scan_rows_kernel, dim3(1), dim3(1024), 0, st, S.row_counts.as<const uint32_t>(), L - 1,
+                     S.row_off.as<uint64_t>(), S.total.as<uint64_t>());
+  LG_HIP(hipGetLastError());
+  if ((rc = S.stage.ensure(64))) return rc;
+  LG_HIP(hipMemcpyAsync(S.stage.p, S.total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  LG_HIP(hipStreamSynchronize(st));
+  const uint64_t n = *S.stage.as<uint64_t>();
+  if (n) {
+    if ((rc = dst.ensure(n * sizeof(double2)))) return rc;
+    lg::StageScope sc(ctx, "sm_cross_write", L * (L - 1));
+    hipLaunchKernelGGL(sm_crossings_kernel<1>, dim3((unsigned)(L - 1)), dim3(kBlock), 0, st,
+                       S.lines.as<const double4>(), (uint32_t)L, S.row_counts.as<uint32_t>(),
+                       S.row_off.as<const uint64_t>(), dst.as<double2>());
+    LG_HIP(hipGetLastError());
+  }
+  *count = n;
+  return LG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lg_string_mod_draw(lg_ctx* ctx, uint32_t n, const lg_string_mod* mods, lg_segments* out) {
+  if (!ctx || !out || (n && !mods)) return fail(LG_E_INVALID, "string mod: ctx, mods or out is NULL");
+  if (out->ctx != ctx) return fail(LG_E_INVALID, "string mod: segments belong to another context");
+  if (!(out->flags & LG_SEG_VERTEX_COLORS))
+    return fail(LG_E_INVALID, "string mod: the segment buffer needs LG_SEG_VERTEX_COLORS");
+  int rc;
+  for (uint32_t i = 0; i < n; i++) {
+    if ((rc = check_mod(mods[i], false))) return rc;
+    if (mods[i].nested) {
+      if ((rc = check_mod(*mods[i].nested, true))) return rc;
+      if (mods[i].modulo > LG_MAX_CROSSING_LINES)
+        return fail(LG_E_INVALID, "string mod: nested over more than LG_MAX_CROSSING_LINES lines");
+    }
+  }
+  LG_HIP(hipSetDevice(ctx->device));
+  lg::begin_call(ctx);
+  hipStream_t st = ctx->stream;
+  lg::SmScratch& S = scratch(ctx);
+  for (lg::DevBuf* b : S.crossings) delete b;
+  S.crossings.clear();
+
+  // nested StringMods draw over the crossing points of their outer lines (string_mod.rs:153-154)
+  std::vector<SmDev> descs;
+  std::vector<lg_mod_rem_color> mrc;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const lg_string_mod& outer = mods[i];
+    const lg_string_mod& m = outer.nested ? *outer.nested : outer;
+    SmDev d = make_dev(m, (uint32_t)mrc.size());
+    mrc.insert(mrc.end(), m.modulo_colors, m.modulo_colors + m.n_modulo_colors);
+    if (outer.nested) {
+      S.crossings.push_back(new lg::DevBuf());
+      uint64_t np = 0;
+      if ((rc = crossings_device(ctx, outer, *S.crossings.back(), &np))) return rc;
+      d.pts = S.crossings.back()->as<const double2>();
+      d.n_points = np;
+    }
+    // draw_init_points returns nothing when there are no points (string_mod.rs:106-108)
+    const uint64_t segs = d.n_points ? m.modulo : 0;
+    if (!segs) continue;
+    d.seg_base = total;
+    total += segs;
+    descs.push_back(d);
+  }
+  if (total >= (1ull << 32)) return fail(LG_E_INVALID, "string mod: 2^32 or more segments in one call");
+  if ((rc = lg::seg_ensure(out, std::max<uint64_t>(total, 1), false))) return rc;
+  float max_abs = 0.0f;
+  if (total) {
+    const size_t db = descs.size() * sizeof(SmDev), mb = std::max<size_t>(mrc.size(), 1) * sizeof(lg_mod_rem_color);
+    if ((rc = S.descs.ensure(db)) || (rc = S.mrc.ensure(mb)) || (rc = S.maxbits.ensure(16)) ||
+        (rc = S.stage.ensure(std::max<size_t>(db + mb, 64))))
+      return rc;
+    char* stage = S.stage.as<char>();
+    std::memcpy(stage, descs.data(), db);
+    if (!mrc.empty()) std::memcpy(stage + db, mrc.data(), mrc.size() * sizeof(lg_mod_rem_color));
+    LG_HIP(hipMemcpyAsync(S.descs.p, stage, db, hipMemcpyHostToDevice, st));
+    LG_HIP(hipMemcpyAsync(S.mrc.p, stage + db, mb, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(sm_zero_u32_kernel, dim3(1), dim3(1), 0, st, S.maxbits.as<unsigned int>());
+    {
+      lg::StageScope sc(ctx, "sm_generate", total);
+      hipLaunchKernelGGL(sm_segments_kernel, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                         S.descs.as<const SmDev>(), (int)descs.size(), S.mrc.as<const lg_mod_rem_color>(), total,
+                         out->pos.as<float4>(), out->col0.as<float4>(), out->col1.as<float4>(),
+                         out->key.as<uint64_t>(), S.maxbits.as<unsigned int>());
+    }
+    LG_HIP(hipGetLastError());
+    // the staging buffer is free again once this copy has landed (the stream is in order)
+    LG_HIP(hipMemcpyAsync(stage, S.maxbits.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipStreamSynchronize(st));
+    unsigned int bits;
+    std::memcpy(&bits, stage, sizeof(bits));
+    std::memcpy(&max_abs, &bits, sizeof(bits));
+  } else {
+    LG_HIP(hipStreamSynchronize(st));
+  }
+  out->count = total;
+  out->max_abs = max_abs;
+  out->has_pos64 = false;
+  lg::end_call(ctx);
+  return LG_OK;
+}
+
+int lg_string_mod_crossings(lg_ctx* ctx, const lg_string_mod* sm, double* xy, uint64_t cap_points,
+                            uint64_t* n_points) {
+  if (!ctx || !sm || !n_points) return fail(LG_E_INVALID, "string mod: ctx, sm or n_points is NULL");
+  int rc;
+  if ((rc = check_mod(*sm, false))) return rc;
+  LG_HIP(hipSetDevice(ctx->device));
+  lg::begin_call(ctx);
+  lg::SmScratch& S = scratch(ctx);
+  lg::DevBuf pts;
+  uint64_t np = 0;
+  if ((rc = crossings_device(ctx, *sm, pts, &np))) return rc;
+  LG_HIP(hipStreamSynchronize(ctx->stream));
+  *n_points = np;
+  (void)S;
+  if (np > cap_points || (np && !xy)) return fail(LG_E_CAPACITY, "string mod: crossing points exceed cap_points");
+  if (np) LG_HIP(hipMemcpy(xy, pts.p, np * sizeof(double2), hipMemcpyDeviceToHost));
+  lg::end_call(ctx);
+  return LG_OK;
+}
+
+}  // extern "C"

@@ -1,15 +1,22 @@
 """StringMod: modular string art (src/light_garden/string_mod.rs:3-188), the second
 producer of segments for the accumulation.
 
-Segments are generated on the host in FP64 like the reference, then uploaded as FP32
-(the reference's vertex conversion, sub_render_pass.rs:188-196).
+draw(ctx, mods, segs) generates every StringMod's segments on the GPU (lg_string_mod_draw:
+all curves and modes, modulo colours, nested StringMods over line crossings);
+crossings(ctx, sm) is line_crossings_as_points (lg_string_mod_crossings). The numpy
+`segments()` below is the host form of the Circle curve, kept to build inputs for the
+upload path (lg_segments_upload) and as a second restatement in tests.
 """
+import ctypes as C
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
+from . import abi
+
 MODE_ADD, MODE_MUL, MODE_POW, MODE_BASE = 0, 1, 2, 3
+CURVE_CIRCLE, CURVE_COMPLEX_EXP, CURVE_HYPOTROCHOID, CURVE_LISSAJOUS = 0, 1, 2, 3
 
 
 @dataclass
@@ -58,7 +65,48 @@ class StringMod:
     turns: int = 1
     mode: int = MODE_MUL
     modulo_colors: List[ModRemColor] = field(default_factory=list)
-    scale: float = 1.0  # Curve::Circle is the unit circle; this build adds a radius
+    scale: float = 1.0  # the reference's curves have radius 1; this build adds a radius
+    curve: int = CURVE_CIRCLE
+    curve_u: Tuple[int, int, int] = (0, 0, 0)  # Hypotrochoid r, s, d; Lissajous a, b
+    curve_f: Tuple[float, float] = (0.0, 0.0)  # ComplexExp c = re + i im; Lissajous delta
+    nested: Optional["StringMod"] = None       # Option<Box<StringMod>>
+
+    @staticmethod
+    def complex_exp(c: complex, **kw) -> "StringMod":
+        return StringMod(curve=CURVE_COMPLEX_EXP, curve_f=(c.real, c.imag), **kw)
+
+    @staticmethod
+    def hypotrochoid(r: int, s: int, d: int, **kw) -> "StringMod":
+        return StringMod(curve=CURVE_HYPOTROCHOID, curve_u=(r, s, d), **kw)
+
+    @staticmethod
+    def lissajous(a: int, b: int, delta: float, **kw) -> "StringMod":
+        return StringMod(curve=CURVE_LISSAJOUS, curve_u=(a, b, 0), curve_f=(delta, 0.0), **kw)
+
+    def to_abi(self):
+        """(lg_string_mod, keep-alive list): the struct points into arrays held by the list."""
+        st = abi.LgStringMod()
+        keep = []
+        st.modulo, st.num, st.turns = int(self.modulo), int(self.num), int(self.turns)
+        st.mode, st.curve = int(self.mode), int(self.curve)
+        st.curve_u[:] = [int(v) for v in self.curve_u]
+        st.curve_f[:] = [float(v) for v in self.curve_f]
+        st.scale = float(self.scale)
+        st.color[:] = [float(c) for c in self.color]
+        if self.modulo_colors:
+            arr = (abi.LgModRemColor * len(self.modulo_colors))()
+            for i, m in enumerate(self.modulo_colors):
+                arr[i].modulo, arr[i].rem = int(m.modulo), int(m.rem)
+                arr[i].color[:] = [float(c) for c in m.color]
+            st.modulo_colors = C.cast(arr, C.POINTER(abi.LgModRemColor))
+            keep.append(arr)
+        st.n_modulo_colors = len(self.modulo_colors)
+        if self.nested is not None:
+            inner, inner_keep = self.nested.to_abi()
+            inner.nested = None  # draw() nests one level (string_mod.rs:152-158)
+            st.nested = C.pointer(inner)
+            keep += [inner, inner_keep]
+        return st, keep
 
     def init_points(self) -> np.ndarray:
         """Curve::Circle points (string_mod.rs:45-54): angle = (turns * n) * TAU / m."""
@@ -76,9 +124,10 @@ class StringMod:
                 return (iix + np.uint64(self.num)) % m
             if self.mode == MODE_MUL:
                 return (iix * np.uint64(self.num)) % m
+            # `self.num as u32` (Pow) and `iix as u32` (Base) truncate before u64::pow
             if self.mode == MODE_POW:
-                return _wrapping_pow(iix, self.num) % m
-            return _wrapping_pow_exps(self.num, iix) % m
+                return _wrapping_pow(iix, int(self.num) & 0xFFFFFFFF) % m
+            return _wrapping_pow_exps(self.num, iix & np.uint64(0xFFFFFFFF)) % m
 
     def get_colors(self, ix: np.ndarray) -> np.ndarray:
         """StringMod::get_color (string_mod.rs:124-150) for an index array."""
@@ -102,3 +151,28 @@ class StringMod:
         ix = self.targets()
         pos = np.concatenate([pts[iix.astype(np.int64)], pts[ix.astype(np.int64)]], axis=1)
         return pos, self.get_colors(iix), self.get_colors(ix)
+
+
+def draw(ctx, mods: Sequence[StringMod], segs) -> int:
+    """LightGarden::draw in Mode::StringMod (mod.rs:681-689) on the GPU: every StringMod's
+    draw() into `segs` (a Segments created with vertex_colors=True). Returns the count."""
+    arr = (abi.LgStringMod * max(1, len(mods)))()
+    keep = []
+    for i, m in enumerate(mods):
+        st, k = m.to_abi()
+        arr[i] = st
+        keep.append(k)
+    abi.call("lg_string_mod_draw", ctx.handle, len(mods), arr, segs.handle)
+    return segs.count()
+
+
+def crossings(ctx, sm: StringMod) -> np.ndarray:
+    """StringMod::line_crossings_as_points (string_mod.rs:87-102) on the GPU: (P, 2) float64."""
+    st, _keep = sm.to_abi()
+    n = C.c_uint64(0)
+    rc = abi.load().lg_string_mod_crossings(ctx.handle, C.byref(st), None, 0, C.byref(n))
+    if rc not in (abi.LG_OK, abi.LG_E_CAPACITY):
+        abi.check(rc, "lg_string_mod_crossings")
+    out = np.empty((max(n.value, 1), 2), np.float64)
+    abi.call("lg_string_mod_crossings", ctx.handle, C.byref(st), out.ctypes.data, n.value, C.byref(n))
+    return out[:n.value]

@@ -1136,7 +1136,7 @@ void or_resolve_f32(const float* accum, uint64_t n_pixels, uint16_t* out) {
 /* ------------------------------------------------------------------------------------ */
 /* string mod (string_mod.rs:33-158)                                                     */
 /* ------------------------------------------------------------------------------------ */
-static uint64_t wrapping_pow(uint64_t b, uint64_t e) {
+static uint64_t wrapping_pow(uint64_t b, uint32_t e) {
   /* u64::pow in a release build wraps modulo 2^64 */
   uint64_t r = 1;
   while (e) {
@@ -1168,25 +1168,182 @@ static void sm_color(uint64_t ix, const float base[4], const or_mod_rem_color* m
 void or_string_mod(uint64_t modulo, uint64_t num, uint64_t turns, int mode, double scale,
                    const float color[4], const or_mod_rem_color* mrc, int n_mrc, double* pos,
                    float* color0, float* color1) {
-  for (uint64_t iix = 0; iix < modulo; iix++) {
-    uint64_t ix;
-    switch (mode) {
-      case OR_SM_ADD: ix = (iix + num) % modulo; break;
-      case OR_SM_MUL: ix = (iix * num) % modulo; break;
-      case OR_SM_POW: ix = wrapping_pow(iix, num) % modulo; break;
-      default: ix = wrapping_pow(num, iix) % modulo; break;
-    }
-    /* Curve::Circle point n: angle = (turns * n) * TAU / m (string_mod.rs:45-54) */
-    double a0 = (double)(turns * iix) * (2.0 * M_PI) / (double)modulo;
-    double a1 = (double)(turns * ix) * (2.0 * M_PI) / (double)modulo;
-    double* p = pos + 4 * iix;
-    p[0] = scale * cos(a0);
-    p[1] = scale * sin(a0);
-    p[2] = scale * cos(a1);
-    p[3] = scale * sin(a1);
-    sm_color(iix, color, mrc, n_mrc, color0 + 4 * iix);
-    sm_color(ix, color, mrc, n_mrc, color1 + 4 * iix);
+  lg_string_mod sm;
+  memset(&sm, 0, sizeof(sm));
+  sm.modulo = modulo;
+  sm.num = num;
+  sm.turns = turns;
+  sm.mode = (uint32_t)mode;
+  sm.curve = LG_CURVE_CIRCLE;
+  sm.scale = scale;
+  memcpy(sm.color, color, sizeof(sm.color));
+  sm.modulo_colors = (const lg_mod_rem_color*)mrc;
+  sm.n_modulo_colors = (uint32_t)n_mrc;
+  or_sm_draw_points(&sm, NULL, modulo, pos, color0, color1);
+}
+
+uint64_t or_sm_target(const lg_string_mod* sm, uint64_t iix) {
+  /* string_mod.rs:111-117; `self.num as u32` and `iix as u32` truncate (Pow, Base) */
+  const uint64_t m = sm->modulo;
+  switch (sm->mode) {
+    case OR_SM_ADD: return (iix + sm->num) % m;
+    case OR_SM_MUL: return (iix * sm->num) % m;
+    case OR_SM_POW: return wrapping_pow(iix, (uint32_t)sm->num) % m;
+    default: return wrapping_pow(sm->num, (uint32_t)iix) % m;
   }
+}
+
+/* num-complex 0.4.6 Complex::powu: square-and-multiply (num-traits' pow), with
+ * (a + bi)(c + di) = (ac - bd) + (ad + bc)i. */
+static void cmul(const double a[2], const double b[2], double out[2]) {
+  const double re = a[0] * b[0] - a[1] * b[1];
+  const double im = a[0] * b[1] + a[1] * b[0];
+  out[0] = re;
+  out[1] = im;
+}
+
+static void cpowu(const double c[2], uint32_t exp, double out[2]) {
+  double base[2] = {c[0], c[1]};
+  if (exp == 0) {
+    out[0] = 1.0;
+    out[1] = 0.0;
+    return;
+  }
+  while ((exp & 1u) == 0) {
+    cmul(base, base, base);
+    exp >>= 1;
+  }
+  if (exp == 1) {
+    out[0] = base[0];
+    out[1] = base[1];
+    return;
+  }
+  double acc[2] = {base[0], base[1]};
+  while (exp > 1) {
+    exp >>= 1;
+    cmul(base, base, base);
+    if (exp & 1u) cmul(acc, base, acc);
+  }
+  out[0] = acc[0];
+  out[1] = acc[1];
+}
+
+void or_sm_point(const lg_string_mod* sm, uint64_t n, double out[2]) {
+  /* string_mod.rs:35-83: angle = (turns * n) as f64 * TAU / m */
+  const double angle = (double)(sm->turns * n) * (2.0 * M_PI) / (double)sm->modulo;
+  double x, y;
+  switch (sm->curve) {
+    case LG_CURVE_COMPLEX_EXP: {
+      double cp[2];
+      cpowu(sm->curve_f, (uint32_t)(sm->turns * n), cp);
+      x = cp[0];
+      y = cp[1];
+      break;
+    }
+    case LG_CURVE_HYPOTROCHOID: {
+      const double small_radius = (double)sm->curve_u[0], big_radius = (double)sm->curve_u[1];
+      const double off_center = (double)sm->curve_u[2];
+      const double smr = big_radius - small_radius;
+      const double ratio = smr + off_center;
+      x = smr * cos(angle) + off_center * cos(angle * smr / small_radius);
+      y = smr * sin(angle) - off_center * sin(angle * smr / small_radius);
+      x /= ratio;
+      y /= ratio;
+      break;
+    }
+    case LG_CURVE_LISSAJOUS:
+      x = sin((double)sm->curve_u[0] * angle + sm->curve_f[0]);
+      y = sin((double)sm->curve_u[1] * angle);
+      break;
+    default:
+      x = cos(angle);
+      y = sin(angle);
+      break;
+  }
+  out[0] = x * sm->scale;
+  out[1] = y * sm->scale;
+}
+
+uint64_t or_sm_draw_points(const lg_string_mod* sm, const double* points, uint64_t n_points, double* pos,
+                           float* color0, float* color1) {
+  /* draw_init_points (string_mod.rs:104-122): nothing when there are no points */
+  if (n_points == 0) return 0;
+  const or_mod_rem_color* mrc = (const or_mod_rem_color*)sm->modulo_colors;
+  for (uint64_t iix = 0; iix < sm->modulo; iix++) {
+    const uint64_t ix = or_sm_target(sm, iix);
+    double* p = pos + 4 * iix;
+    if (points) {
+      const uint64_t a = iix % n_points, b = ix % n_points;
+      p[0] = points[2 * a];
+      p[1] = points[2 * a + 1];
+      p[2] = points[2 * b];
+      p[3] = points[2 * b + 1];
+    } else {
+      or_sm_point(sm, iix, p);
+      or_sm_point(sm, ix, p + 2);
+    }
+    sm_color(iix, sm->color, mrc, (int)sm->n_modulo_colors, color0 + 4 * iix);
+    sm_color(ix, sm->color, mrc, (int)sm->n_modulo_colors, color1 + 4 * iix);
+  }
+  return sm->modulo;
+}
+
+/* DEFINED (collision2d's LineSegment::intersect is absent): a + t r = c + u s with r = b - a,
+ * s = d - c, t = ((c - a) x s) / (r x s), u = ((c - a) x r) / (r x s); a crossing iff
+ * r x s != 0 and t, u in [0, 1]; the point is a + t r. */
+static int segment_crossing(const double* A, const double* B, double out[2]) {
+  const double rx = A[2] - A[0], ry = A[3] - A[1];
+  const double sx = B[2] - B[0], sy = B[3] - B[1];
+  const double denom = rx * sy - ry * sx;
+  if (denom == 0.0) return 0;
+  const double qx = B[0] - A[0], qy = B[1] - A[1];
+  const double t = (qx * sy - qy * sx) / denom;
+  const double u = (qx * ry - qy * rx) / denom;
+  if (!(t >= 0.0 && t <= 1.0 && u >= 0.0 && u <= 1.0)) return 0;
+  out[0] = A[0] + t * rx;
+  out[1] = A[1] + t * ry;
+  return 1;
+}
+
+uint64_t or_line_crossings(const double* lines, uint64_t n_lines, double* out, uint64_t cap) {
+  /* line_crossings_as_points (string_mod.rs:87-102): diff = 1..L outer, ixa = 0..L inner */
+  uint64_t n = 0;
+  for (uint64_t diff = 1; diff < n_lines; diff++) {
+    for (uint64_t ixa = 0; ixa < n_lines; ixa++) {
+      double p[2];
+      if (segment_crossing(lines + 4 * ixa, lines + 4 * ((ixa + diff) % n_lines), p)) {
+        if (n < cap) {
+          out[2 * n] = p[0];
+          out[2 * n + 1] = p[1];
+        }
+        n++;
+      }
+    }
+  }
+  return n;
+}
+
+uint64_t or_sm_crossings(const lg_string_mod* sm, double* out, uint64_t cap) {
+  const uint64_t L = sm->modulo;
+  double* lines = (double*)malloc((size_t)(L ? L : 1) * 4 * sizeof(double));
+  for (uint64_t i = 0; i < L; i++) {
+    or_sm_point(sm, i, lines + 4 * i);
+    or_sm_point(sm, or_sm_target(sm, i), lines + 4 * i + 2);
+  }
+  const uint64_t n = or_line_crossings(lines, L, out, cap);
+  free(lines);
+  return n;
+}
+
+uint64_t or_sm_draw(const lg_string_mod* sm, double* pos, float* color0, float* color1) {
+  /* StringMod::draw (string_mod.rs:152-158) */
+  if (!sm->nested) return or_sm_draw_points(sm, NULL, sm->modulo, pos, color0, color1);
+  const uint64_t n = or_sm_crossings(sm, NULL, 0);
+  double* pts = (double*)malloc((size_t)(n ? n : 1) * 2 * sizeof(double));
+  or_sm_crossings(sm, pts, n);
+  const uint64_t segs = or_sm_draw_points(sm->nested, pts, n, pos, color0, color1);
+  free(pts);
+  return segs;
 }
 
 double or_now_seconds(void) {

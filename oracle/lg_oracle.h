@@ -104,6 +104,20 @@ void or_string_mod(uint64_t modulo, uint64_t num, uint64_t turns, int mode, doub
                    const float color[4], const or_mod_rem_color* mrc, int n_mrc,
                    double* pos, float* color0, float* color1);
 
+/* General StringMod (every curve and mode; lg_string_mod as the library takes it). */
+uint64_t or_sm_target(const lg_string_mod* sm, uint64_t iix);          /* string_mod.rs:111-117 */
+void or_sm_point(const lg_string_mod* sm, uint64_t n, double out[2]);  /* init_points, 35-83 */
+/* draw_init_points (104-122) over `points` (2 doubles each; NULL = the curve's own points);
+ * writes sm->modulo segments, none when n_points is 0; returns the count. */
+uint64_t or_sm_draw_points(const lg_string_mod* sm, const double* points, uint64_t n_points,
+                           double* pos, float* color0, float* color1);
+/* line_crossings_as_points (87-102) over explicit lines {ax, ay, bx, by}; writes at most cap
+ * points, returns the count. DEFINED segment crossing: see lg_oracle.c. */
+uint64_t or_line_crossings(const double* lines, uint64_t n_lines, double* out, uint64_t cap);
+uint64_t or_sm_crossings(const lg_string_mod* sm, double* out, uint64_t cap);
+/* draw (152-158): with sm->nested, the nested StringMod over sm's crossing points. */
+uint64_t or_sm_draw(const lg_string_mod* sm, double* pos, float* color0, float* color1);
+
 /* ---- timing helper for the CPU baseline ------------------------------------------- */
 double or_now_seconds(void);
 

@@ -62,6 +62,12 @@ def load():
         "or_double_from_half": (C.c_double, [C.c_uint16]),
         "or_string_mod": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_double, P,
                                  C.POINTER(ModRemColor), C.c_int, P, P, P]),
+        "or_sm_target": (C.c_uint64, [P, C.c_uint64]),
+        "or_sm_point": (None, [P, C.c_uint64, D2]),
+        "or_sm_draw_points": (C.c_uint64, [P, P, C.c_uint64, P, P, P]),
+        "or_line_crossings": (C.c_uint64, [P, C.c_uint64, P, C.c_uint64]),
+        "or_sm_crossings": (C.c_uint64, [P, P, C.c_uint64]),
+        "or_sm_draw": (C.c_uint64, [P, P, P, P]),
         "or_now_seconds": (C.c_double, []),
     }
     for name, (res, args) in sig.items():
@@ -207,6 +213,67 @@ def string_mod(modulo, num, turns=1, mode=1, scale=1.0, color=(1, 1, 1, 1), mrc=
     load().or_string_mod(modulo, num, turns, mode, scale, col, arr, len(mrc), pos.ctypes.data,
                          c0.ctypes.data, c1.ctypes.data)
     return pos, c0, c1
+
+
+def _sm_ref(sm):
+    """(pointer, keep-alive) of a light_garden_amd.string_mod.StringMod's lg_string_mod."""
+    st, keep = sm.to_abi()
+    return C.byref(st), (st, keep)
+
+
+def sm_target(sm, iix):
+    ref, _k = _sm_ref(sm)
+    return load().or_sm_target(ref, int(iix))
+
+
+def sm_point(sm, n):
+    ref, _k = _sm_ref(sm)
+    out = (C.c_double * 2)()
+    load().or_sm_point(ref, int(n), out)
+    return out[0], out[1]
+
+
+def sm_draw_points(sm, points):
+    """draw_init_points over explicit points (n, 2) float64."""
+    ref, _k = _sm_ref(sm)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
+    m = int(sm.modulo) if len(pts) else 0
+    pos = np.empty((max(m, 1), 4), np.float64)
+    c0 = np.empty((max(m, 1), 4), np.float32)
+    c1 = np.empty((max(m, 1), 4), np.float32)
+    n = load().or_sm_draw_points(ref, pts.ctypes.data if len(pts) else None, len(pts), pos.ctypes.data,
+                                 c0.ctypes.data, c1.ctypes.data)
+    return pos[:n], c0[:n], c1[:n]
+
+
+def line_crossings(lines):
+    """line_crossings_as_points over explicit lines (L, 4) = (ax, ay, bx, by)."""
+    ln = np.ascontiguousarray(lines, np.float64).reshape(-1, 4)
+    lib = load()
+    n = lib.or_line_crossings(ln.ctypes.data, len(ln), None, 0)
+    out = np.empty((max(n, 1), 2), np.float64)
+    lib.or_line_crossings(ln.ctypes.data, len(ln), out.ctypes.data, n)
+    return out[:n]
+
+
+def sm_crossings(sm):
+    ref, _k = _sm_ref(sm)
+    lib = load()
+    n = lib.or_sm_crossings(ref, None, 0)
+    out = np.empty((max(n, 1), 2), np.float64)
+    lib.or_sm_crossings(ref, out.ctypes.data, n)
+    return out[:n]
+
+
+def sm_draw(sm):
+    """StringMod::draw: (pos (S, 4) float64, color0, color1 (S, 4) float32)."""
+    ref, _k = _sm_ref(sm)
+    m = int(sm.nested.modulo if sm.nested is not None else sm.modulo)
+    pos = np.empty((max(m, 1), 4), np.float64)
+    c0 = np.empty((max(m, 1), 4), np.float32)
+    c1 = np.empty((max(m, 1), 4), np.float32)
+    n = load().or_sm_draw(ref, pos.ctypes.data, c0.ctypes.data, c1.ctypes.data)
+    return pos[:n], c0[:n], c1[:n]
 
 
 def now():

@@ -1,0 +1,164 @@
+"""GPU StringMod generator (lg_string_mod_draw) and line crossings (lg_string_mod_crossings)
+against the FP64 oracle (string_mod.rs:33-158)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import light_garden_amd as lg
+from light_garden_amd import abi, configs
+from light_garden_amd import string_mod as S
+from light_garden_amd.string_mod import ModRemColor, StringMod
+
+pytestmark = pytest.mark.gpu
+
+MRC = [ModRemColor(3, 1, [1.0, 0.0, 0.0, 1.0]), ModRemColor(5, 2, [0.0, 1.0, 0.0, 0.5]),
+       ModRemColor(7, 0, [0.0, 0.0, 1.0, 0.25])]
+TH = 2 * math.pi / 89
+UNIT = complex(math.cos(TH), math.sin(TH))
+
+
+def curves(m):
+    kw = dict(modulo=m, color=[0.2, 0.3, 0.4, 0.5], modulo_colors=MRC, scale=0.9)
+    return {
+        "circle": lambda **k: StringMod(turns=2, **kw, **k),
+        "complex_exp": lambda **k: StringMod.complex_exp(UNIT * 1.0001, turns=1, **kw, **k),
+        "hypotrochoid": lambda **k: StringMod.hypotrochoid(3, 8, 2, turns=1, **kw, **k),
+        "lissajous": lambda **k: StringMod.lissajous(3, 2, 0.7, turns=3, **kw, **k),
+    }
+
+
+def gpu_draw(ctx, mods):
+    segs = lg.Segments(ctx, 16, vertex_colors=True)
+    n = S.draw(ctx, mods, segs)
+    pos, c0, c1, key = segs.download()
+    segs.close()
+    return n, pos, c0, c1, key
+
+
+def f32_ulps(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.abs(a.astype(np.float64) - b.astype(np.float64)) / np.spacing(np.maximum(np.abs(a), np.abs(b)))
+
+
+@pytest.mark.parametrize("curve", ["circle", "complex_exp", "hypotrochoid", "lissajous"])
+@pytest.mark.parametrize("mode", [S.MODE_ADD, S.MODE_MUL, S.MODE_POW, S.MODE_BASE])
+def test_draw_matches_oracle(ctx, orc, curve, mode):
+    sm = curves(1999)[curve](num=13, mode=mode)
+    n, pos, c0, c1, key = gpu_draw(ctx, [sm])
+    rpos, rc0, rc1 = orc.sm_draw(sm)
+    assert n == 1999
+    np.testing.assert_array_equal(key, np.arange(n))
+    np.testing.assert_array_equal(c0, rc0)
+    np.testing.assert_array_equal(c1, rc1)
+    want = rpos.astype(np.float32)
+    if curve == "complex_exp":  # no libm: the same FP64 operations, bit for bit
+        np.testing.assert_array_equal(pos, want)
+    else:  # the device's sin/cos against the host libm's: at most one FP32 step apart
+        assert f32_ulps(pos, want).max() <= 1.0
+
+
+def test_crossings_complex_exp_bit_exact(ctx, orc):
+    sm = StringMod.complex_exp(UNIT, modulo=89 * 3, turns=1, num=5)
+    got = S.crossings(ctx, sm)
+    want = orc.sm_crossings(sm)
+    assert len(want) > 10000
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("m,k", [(101, 2), (509, 7), (1024, 3)])
+def test_crossings_circle(ctx, orc, m, k):
+    sm = StringMod(modulo=m, num=k)
+    got = S.crossings(ctx, sm)
+    want = orc.sm_crossings(sm)
+    assert len(got) == len(want) > 0
+    np.testing.assert_allclose(got, want, atol=1e-12, rtol=0)
+
+
+def test_crossings_tiny_and_empty(ctx, orc):
+    for m in (0, 1, 2, 3):
+        sm = StringMod(modulo=m, num=2)
+        got = S.crossings(ctx, sm)
+        assert len(got) == len(orc.sm_crossings(sm))
+
+
+def test_nested_draw_matches_oracle(ctx, orc):
+    inner = StringMod(modulo=4000, num=17, mode=S.MODE_MUL, color=[0.1, 0.2, 0.3, 0.4], modulo_colors=MRC[:2])
+    outer = StringMod.complex_exp(UNIT, modulo=89 * 2, num=3, nested=inner)
+    n, pos, c0, c1, key = gpu_draw(ctx, [outer])
+    rpos, rc0, rc1 = orc.sm_draw(outer)
+    assert n == len(rpos) == 4000
+    np.testing.assert_array_equal(pos, rpos.astype(np.float32))
+    np.testing.assert_array_equal(c0, rc0)
+    np.testing.assert_array_equal(c1, rc1)
+
+
+def test_several_mods_concatenate(ctx, orc):
+    c = curves(300)
+    mods = [c["circle"](num=3), StringMod(modulo=0), c["lissajous"](num=5, mode=S.MODE_ADD),
+            StringMod(modulo=1, nested=StringMod(modulo=10)),  # no crossings: draws nothing
+            StringMod.complex_exp(UNIT, modulo=89, num=2, nested=StringMod(modulo=77, num=5))]
+    n, pos, c0, c1, key = gpu_draw(ctx, mods)
+    parts = [orc.sm_draw(m) for m in mods]
+    rpos = np.concatenate([p[0] for p in parts])
+    assert n == len(rpos) == 300 + 300 + 77
+    assert f32_ulps(pos, rpos.astype(np.float32)).max() <= 1.0
+    np.testing.assert_array_equal(c0, np.concatenate([p[1] for p in parts]))
+    np.testing.assert_array_equal(c1, np.concatenate([p[2] for p in parts]))
+    np.testing.assert_array_equal(key, np.arange(n))
+
+
+def test_generated_equals_uploaded_image(ctx):
+    """Generated segments accumulate to the same bits as the same segments uploaded (the
+    generator reports the exact colour bound, as the upload computes it)."""
+    spec = configs.StringModSpec(modulo=20000, mults=(2, 3, 5), width=256, height=256)
+    mods = [StringMod(modulo=spec.modulo, num=k, color=c, scale=spec.scale) for k, c in zip(spec.mults, spec.colors())]
+    gen = lg.Segments(ctx, 16, vertex_colors=True)
+    n = S.draw(ctx, mods, gen)
+    pos, c0, c1, _ = gen.download()
+    up = lg.Segments(ctx, n, vertex_colors=True)
+    up.upload(pos, c0, c1)
+    imgs = []
+    for segs in (gen, up):
+        img = lg.Image(ctx, spec.width, spec.height)
+        img.accumulate(segs)
+        img.resolve()
+        imgs.append(img.rgba16f())
+        img.close()
+    gen.close()
+    up.close()
+    assert imgs[0].any()
+    np.testing.assert_array_equal(imgs[0], imgs[1])
+
+
+def test_string_mod_workload_matches_host_form(ctx):
+    """The 16 Mul-mode StringMods of the string-mod accumulation, generated on the GPU, against
+    the host form that the upload path uses (numpy, FP64)."""
+    spec = configs.StringModSpec(modulo=100_000)
+    mods = [StringMod(modulo=spec.modulo, num=k, color=c, scale=spec.scale) for k, c in zip(spec.mults, spec.colors())]
+    n, pos, c0, c1, key = gpu_draw(ctx, mods)
+    hpos, hcol = configs.string_mod_segments(spec)
+    assert n == len(hpos) == 1_600_000
+    assert f32_ulps(pos, hpos.astype(np.float32)).max() <= 1.0
+    np.testing.assert_array_equal(c0, hcol)
+    np.testing.assert_array_equal(c1, hcol)
+
+
+def test_rejects_bad_string_mods(ctx):
+    lib = abi.load()
+    plain = lg.Segments(ctx, 16)
+    vc = lg.Segments(ctx, 16, vertex_colors=True)
+    cases = [
+        (StringMod(modulo=10), plain),                                       # no per-vertex colours
+        (StringMod(modulo=10, modulo_colors=[ModRemColor(0, 0, [1, 1, 1, 1])]), vc),  # modulo 0
+        (StringMod(modulo=10, curve=9), vc),
+        (StringMod(modulo=10, mode=7), vc),
+        (StringMod(modulo=abi.MAX_CROSSING_LINES + 1, nested=StringMod(modulo=3)), vc),
+    ]
+    for sm, segs in cases:
+        st, keep = sm.to_abi()
+        assert lib.lg_string_mod_draw(ctx.handle, 1, C.byref(st), segs.handle) == abi.LG_E_INVALID
+    plain.close()
+    vc.close()
