@@ -549,6 +549,76 @@ def bench_string_mod(args):
     print(json.dumps(result))
 
 
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64
+FLOPS_PER_PAIR = 13             # denom, c - a, two cross products, two divisions
+
+
+def bench_crossings(args):
+    """A nested StringMod frame (string_mod.rs:152-158): the outer Circle/Mul StringMod's lines
+    (L = 20,000, k = 7) are crossed pairwise (line_crossings_as_points, L(L-1) = 4.0e8 ordered
+    pairs) and the nested StringMod (modulo 100,000, k = 3) draws over the crossing points, all
+    by lg_string_mod_draw. The unit is line pairs tested per second."""
+    import light_garden_amd as lg
+    from light_garden_amd import string_mod as S
+    L, k = 20_000, 7
+    inner = S.StringMod(modulo=100_000, num=3, color=[0.002, 0.002, 0.002, 0.002])
+    outer = S.StringMod(modulo=L, num=k, scale=0.9, nested=inner)
+    ctx = lg.Context(0)
+    segs = lg.Segments(ctx, capacity=inner.modulo, vertex_colors=True)
+    for _ in range(args.warmup):
+        S.draw(ctx, [outer], segs)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    stage_sum, lat = {}, []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        S.draw(ctx, [outer], segs)
+        add_stage_times(ctx, stage_sum)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    stages = stage_table(stage_sum, args.steps)
+    pairs = L * (L - 1)
+    n_cross = len(S.crossings(ctx, outer))
+    cc = stages["sm_cross_count"]
+    t_cc = cc["avg_launch_ms"] * 1e-3
+    achieved = FLOPS_PER_PAIR * pairs / t_cc / 1e12
+    result = {
+        "metric": "line pairs/sec (1 GPU), nested StringMod line_crossings_as_points", "value": pairs * args.steps / elapsed,
+        "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "crossings: outer Circle Mul L=%d k=%d radius 0.9, %d ordered pairs, %d crossing "
+                               "points; nested Mul modulo 100000 k=3" % (L, k, pairs, n_cross),
+                   "pairs": pairs, "crossings": n_cross},
+        "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "stages": stages,
+        "roofline": {"bound": "valu-fp64", "kernel": "sm_crossings_kernel<0>", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "algorithmic_flops": FLOPS_PER_PAIR * pairs, "avg_launch_ms": cc["avg_launch_ms"]},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        import ctypes as C
+        import numpy as np
+        lib = orc.load()
+        st, keep = outer.to_abi()
+        n_ref = lib.or_sm_crossings(C.byref(st), None, 0)  # count pass, untimed
+        buf = np.empty((max(n_ref, 1), 2), np.float64)
+        t1 = time.perf_counter()
+        lib.or_sm_crossings(C.byref(st), buf.ctypes.data, n_ref)  # the reference's one pass
+        t = time.perf_counter() - t1
+        result["crossings_equal_count_cpu"] = bool(n_ref == n_cross)
+        del buf
+        result["cpu_baseline"] = {"value": pairs / t, "unit": "pairs/s", "cores": 1, "kind": "port",
+                                  "sample": "line_crossings_as_points of the same outer StringMod (%d ordered pairs, "
+                                            "one pass writing every point) by the CPU restatement on one thread, "
+                                            "%.2f s" % (pairs, t)}
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,7 +629,7 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=3, help="CPU baseline traces every n-th ray")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", default="large_single",
-                    choices=["large_single", "plain_small", "string_mod", "batched", "serving"])
+                    choices=["large_single", "plain_small", "string_mod", "batched", "serving", "crossings"])
     ap.add_argument("--requests", type=int, default=1024, help="serving: requests per load level")
     ap.add_argument("--hbm-budget-gib", type=int, default=64, help="serving: HBM budget")
     ap.add_argument("--scenes", type=int, default=256, help="batched: number of scenes")
@@ -585,6 +655,8 @@ def main():
         raise SystemExit("bench.py: %s is not built; run __graft_entry__.build()" % abi.LIB_PATH)
     if args.workload == "string_mod":
         bench_string_mod(args)
+    elif args.workload == "crossings":
+        bench_crossings(args)
     elif args.workload == "plain_small":
         scene = configs.plain_small()
         bench_scene(args, scene, "plain_small: one white point light at the origin, 20000 rays; circle, rect, "

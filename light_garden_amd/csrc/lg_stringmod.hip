@@ -44,8 +44,8 @@ struct SmDev {          // one StringMod as the generator reads it
 };
 
 struct SmScratch {
-  DevBuf descs, mrc, lines, row_counts, row_off, total, maxbits;
-  std::vector<DevBuf*> crossings;  // crossing points of nested StringMods, per call
+  DevBuf descs, mrc, lines, chunk_counts, masks, row_counts, row_off, total, maxbits;
+  std::vector<DevBuf*> crossings;  // crossing points of the i-th nested StringMod of a call (kept, grown)
   PinnedBuf stage;
   ~SmScratch() {
     for (DevBuf* b : crossings) delete b;
@@ -212,74 +212,137 @@ __global__ void __launch_bounds__(kBlock) sm_lines_kernel(SmDev s, double4* __re
 
 // LineSegment::intersect, restated (collision2d is absent): a + t r = c + u s with
 // t = ((c - a) x s) / (r x s), u = ((c - a) x r) / (r x s); a point iff r x s != 0 and
-// t, u in [0, 1]; the point is a + t r. Parallel and zero-length lines never meet.
-__device__ inline bool crossing(const double4 A, const double4 B, double2* out) {
+// t, u in [0, 1] (as the divided values); the point is a + t r. Parallel and zero-length
+// lines never meet.
+//
+// The test runs without a division and without branches: with d = |r x s| and n the
+// numerator signed as if d > 0, fl(n / d) lies in [0, 1] iff n == 0 or (n > 0, n <= 2d and
+// n - d <= d 2^-53): fl(q) <= 1 iff q <= 1 + 2^-53 (round to nearest even), n - d is exact
+// for d < n <= 2d (Sterbenz) and d 2^-53 is exact. That holds while the quotient cannot
+// underflow or the products overflow (d, |n| in [2^-500, 2^500] or n == 0); a lane outside
+// that range (or with NaN/inf) takes the divided test instead.
+__device__ inline bool unit_fast(double n, double d, double two_d, double eps) {
+  return (n == 0.0) | ((n > 0.0) & (n <= two_d) & (n - d <= eps));
+}
+__device__ inline bool in_range(double a) { return (a == 0.0) | ((a >= 0x1p-500) & (a <= 0x1p500)); }
+
+__device__ inline bool crosses(const double4 A, const double4 B) {
   const double denom = A.z * B.w - A.w * B.z;
-  if (denom == 0.0) return false;
+  const double qx = B.x - A.x, qy = B.y - A.y;
+  const double tn = qx * B.w - qy * B.z;
+  const double un = qx * A.w - qy * A.z;
+  const bool neg = denom < 0.0;
+  const double d = fabs(denom);
+  const double t = neg ? -tn : tn, u = neg ? -un : un;
+  const double two_d = d + d, eps = d * 0x1p-53;
+  const bool ok = (d >= 0x1p-500) & (d <= 0x1p500) & in_range(fabs(t)) & in_range(fabs(u));
+  bool hit = ok & unit_fast(t, d, two_d, eps) & unit_fast(u, d, two_d, eps);
+  if (__builtin_expect(!ok & (denom != 0.0), 0)) {
+    const double tq = tn / denom, uq = un / denom;
+    hit = (tq >= 0.0) & (tq <= 1.0) & (uq >= 0.0) & (uq <= 1.0);
+  }
+  return hit;
+}
+
+// The crossing point of two lines already known to cross (t = tn / denom as above).
+__device__ inline double2 crossing_point(const double4 A, const double4 B) {
+  const double denom = A.z * B.w - A.w * B.z;
   const double qx = B.x - A.x, qy = B.y - A.y;
   const double t = (qx * B.w - qy * B.z) / denom;
-  const double u = (qx * A.w - qy * A.z) / denom;
-  if (!(t >= 0.0 && t <= 1.0 && u >= 0.0 && u <= 1.0)) return false;
-  *out = make_double2(A.x + t * A.z, A.y + t * A.w);
-  return true;
+  return make_double2(A.x + t * A.z, A.y + t * A.w);
 }
 
-// Block-wide exclusive prefix of a per-thread flag; returns the block total.
-__device__ inline uint32_t block_prefix(bool f, uint32_t* lds, uint32_t* prefix) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned long long bal = __ballot(f);
-  const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-  if (lane == 0) lds[wave] = (uint32_t)__popcll(bal);
-  __syncthreads();
-  uint32_t before = 0, total = 0;
-  for (int w = 0; w < kBlock / 64; w++) {
-    const uint32_t c = lds[w];
-    before += w < wave ? c : 0u;
-    total += c;
-  }
-  __syncthreads();
-  *prefix = before + below;
-  return total;
-}
+// The pair sweep in blocks of kBlock lines (chunk c = blockIdx.x, ixa = c kBlock + lane) by
+// kRows pair offsets (rows diff = d0 .. d0 + kRows - 1, d0 = 1 + blockIdx.y kRows): each
+// block keeps its lines in registers and the kBlock + kRows - 1 partner lines
+// lines[(ixa + diff) % L] in LDS, so a line is read from memory about twice per kRows pairs.
+// PASS 0 classifies every pair, keeps each lane's kRows hit bits in masks and writes the
+// crossings of every (row, chunk) to counts[(diff - 1) n_chunks + c]. PASS 1 reads the masks
+// back and writes each crossing point at row_off[diff - 1] + chunk_off[(diff - 1) n_chunks + c]
+// + its rank in the chunk: the reference's order (diff outer, ixa inner).
+constexpr int kRows = 64;  // one bit of a per-lane u64 hit mask per row
+constexpr int kWin = kBlock + kRows - 1;
 
-// Row `diff` = blockIdx.x + 1 of the pair sweep. PASS 0 counts the row's crossings into
-// row_counts; PASS 1 writes them at row_off[row] in ixa order.
 template <int PASS>
 __global__ void __launch_bounds__(kBlock) sm_crossings_kernel(const double4* __restrict__ lines, uint32_t L,
-                                                              uint32_t* __restrict__ row_counts,
+                                                              uint32_t n_chunks, uint32_t* __restrict__ counts,
+                                                              unsigned long long* __restrict__ masks,
                                                               const uint64_t* __restrict__ row_off,
                                                               double2* __restrict__ out) {
-  __shared__ uint32_t lds[kBlock / 64];
-  const uint32_t diff = blockIdx.x + 1;
-  uint64_t base = PASS ? row_off[blockIdx.x] : 0;
-  uint32_t count = 0;
-  for (uint32_t c0 = 0; c0 < L; c0 += kBlock) {
-    const uint32_t ixa = c0 + threadIdx.x;
-    bool hit = false;
-    double2 p;
-    if (ixa < L) {
-      uint32_t j = ixa + diff;
-      if (j >= L) j -= L;
-      hit = crossing(lines[ixa], lines[j], &p);
-    }
-    if (PASS == 0) {
-      count += __popcll(__ballot(hit)) * (uint32_t)((threadIdx.x & 63) == 0);
-    } else {
-      uint32_t pre;
-      const uint32_t tot = block_prefix(hit, lds, &pre);
-      if (hit) out[base + pre] = p;
-      base += tot;
-    }
-  }
+  __shared__ double4 win[kWin];
+  __shared__ uint32_t wcnt[kRows][kBlock / 64];
+  const uint32_t c = blockIdx.x;
+  const uint32_t d0 = 1 + blockIdx.y * kRows;
+  const uint32_t w0 = c * kBlock + d0;
+  for (uint32_t k = threadIdx.x; k < (uint32_t)kWin; k += kBlock) win[k] = lines[(w0 + k) % L];
+  const uint32_t ixa = c * kBlock + threadIdx.x;
+  const bool valid = ixa < L;
+  const double4 A = lines[valid ? ixa : 0];
+  unsigned long long* my_mask = masks + ((uint64_t)blockIdx.y * n_chunks + c) * kBlock + threadIdx.x;
+  const uint32_t nr = L - d0 < (uint32_t)kRows ? L - d0 : (uint32_t)kRows;  // rows d0 .. L-1
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  unsigned long long mask = 0;
   if (PASS == 0) {
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = count;
+    for (uint32_t r = 0; r < nr; r++) {
+      const bool hit = valid & crosses(A, win[threadIdx.x + r]);
+      mask |= (unsigned long long)hit << r;
+      const unsigned long long bal = __ballot(hit);
+      if (lane == 0) wcnt[r][wave] = (uint32_t)__popcll(bal);
+    }
+    *my_mask = mask;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < nr) {
       uint32_t t = 0;
-      for (int w = 0; w < kBlock / 64; w++) t += lds[w];
-      row_counts[blockIdx.x] = t;
+      for (int w = 0; w < kBlock / 64; w++) t += wcnt[threadIdx.x][w];
+      counts[(uint64_t)(d0 - 1 + threadIdx.x) * n_chunks + c] = t;
+    }
+    return;
+  }
+  mask = *my_mask;
+  for (uint32_t r = 0; r < nr; r++) {
+    const unsigned long long bal = __ballot((mask >> r) & 1ull);
+    if (lane == 0) wcnt[r][wave] = (uint32_t)__popcll(bal);
+  }
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (uint32_t r = 0; r < nr; r++) {
+    const bool hit = (mask >> r) & 1ull;
+    const unsigned long long bal = __ballot(hit);
+    if (hit) {
+      uint32_t before = 0;
+      for (int w = 0; w < wave; w++) before += wcnt[r][w];
+      const uint64_t row = d0 - 1 + r;
+      const uint64_t base = row_off[row] + counts[row * n_chunks + c];
+      out[base + before + (uint32_t)__popcll(bal & below)] = crossing_point(A, win[threadIdx.x + r]);
     }
   }
+}
+
+// In-place exclusive scan of each row's n_chunks counts (one wave per row); row totals to
+// row_tot.
+__global__ void __launch_bounds__(kBlock) sm_scan_chunks_kernel(uint32_t* __restrict__ counts, uint32_t n_rows,
+                                                                uint32_t n_chunks, uint32_t* __restrict__ row_tot) {
+  const uint32_t row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n_rows) return;
+  uint32_t* v = counts + (uint64_t)row * n_chunks;
+  const uint32_t per = (n_chunks + 63) / 64;
+  const uint32_t b = lane * per, e = b + per < n_chunks ? b + per : n_chunks;
+  uint32_t sum = 0;
+  for (uint32_t i = b; i < e; i++) sum += v[i];
+  uint32_t inc = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)inc, off, 64);
+    if (lane >= off) inc += o;
+  }
+  uint32_t run = inc - sum;
+  for (uint32_t i = b; i < e; i++) {
+    const uint32_t x = v[i];
+    v[i] = run;
+    run += x;
+  }
+  if (lane == 63) row_tot[row] = inc;
 }
 
 // Exclusive prefix of n row counts into u64 offsets and their total (one block).
@@ -354,11 +417,17 @@ int crossings_device(lg_ctx* ctx, const lg_string_mod& outer, lg::DevBuf& dst, u
   if (L > LG_MAX_CROSSING_LINES)
     return fail(LG_E_INVALID, "string mod: line_crossings_as_points over more than LG_MAX_CROSSING_LINES lines");
   if (L < 2) return LG_OK;
+  const uint32_t n_rows = (uint32_t)(L - 1);
+  const uint32_t n_chunks = (uint32_t)((L + kBlock - 1) / kBlock);
   int rc;
-  if ((rc = S.lines.ensure(L * sizeof(double4))) || (rc = S.row_counts.ensure((L - 1) * sizeof(uint32_t))) ||
-      (rc = S.row_off.ensure((L - 1) * sizeof(uint64_t))) || (rc = S.total.ensure(sizeof(uint64_t))))
+  if ((rc = S.lines.ensure(L * sizeof(double4))) ||
+      (rc = S.chunk_counts.ensure((uint64_t)n_rows * n_chunks * sizeof(uint32_t))) ||
+      (rc = S.masks.ensure((uint64_t)((n_rows + kRows - 1) / kRows) * n_chunks * kBlock * 8)) ||
+      (rc = S.row_counts.ensure(n_rows * sizeof(uint32_t))) || (rc = S.row_off.ensure(n_rows * sizeof(uint64_t))) ||
+      (rc = S.total.ensure(sizeof(uint64_t))))
     return rc;
   const SmDev d = make_dev(outer, 0);
+  const dim3 grid(n_chunks, (n_rows + kRows - 1) / kRows);
   {
     lg::StageScope sc(ctx, "sm_lines", L);
     hipLaunchKernelGGL(sm_lines_kernel, dim3((unsigned)((L + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d,
@@ -366,12 +435,17 @@ int crossings_device(lg_ctx* ctx, const lg_string_mod& outer, lg::DevBuf& dst, u
   }
   {
     lg::StageScope sc(ctx, "sm_cross_count", L * (L - 1));
-    hipLaunchKernelGGL(sm_crossings_kernel<0>, dim3((unsigned)(L - 1)), dim3(kBlock), 0, st,
-                       S.lines.as<const double4>(), (uint32_t)L, S.row_counts.as<uint32_t>(),
+    hipLaunchKernelGGL(sm_crossings_kernel<0>, grid, dim3(kBlock), 0, st, S.lines.as<const double4>(), (uint32_t)L,
+                       n_chunks, S.chunk_counts.as<uint32_t>(), S.masks.as<unsigned long long>(),
                        (const uint64_t*)nullptr, (double2*)nullptr);
   }
-  hipLaunchKernelGGL(sm_scan_rows_kernel, dim3(1), dim3(1024), 0, st, S.row_counts.as<const uint32_t>(), L - 1,
-                     S.row_off.as<uint64_t>(), S.total.as<uint64_t>());
+  {
+    lg::StageScope sc(ctx, "sm_cross_scan", (uint64_t)n_rows * n_chunks);
+    hipLaunchKernelGGL(sm_scan_chunks_kernel, dim3((n_rows + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, st,
+                       S.chunk_counts.as<uint32_t>(), n_rows, n_chunks, S.row_counts.as<uint32_t>());
+    hipLaunchKernelGGL(sm_scan_rows_kernel, dim3(1), dim3(1024), 0, st, S.row_counts.as<const uint32_t>(),
+                       (uint64_t)n_rows, S.row_off.as<uint64_t>(), S.total.as<uint64_t>());
+  }
   LG_HIP(hipGetLastError());
   if ((rc = S.stage.ensure(64))) return rc;
   LG_HIP(hipMemcpyAsync(S.stage.p, S.total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -380,8 +454,8 @@ int crossings_device(lg_ctx* ctx, const lg_string_mod& outer, lg::DevBuf& dst, u
   if (n) {
     if ((rc = dst.ensure(n * sizeof(double2)))) return rc;
     lg::StageScope sc(ctx, "sm_cross_write", L * (L - 1));
-    hipLaunchKernelGGL(sm_crossings_kernel<1>, dim3((unsigned)(L - 1)), dim3(kBlock), 0, st,
-                       S.lines.as<const double4>(), (uint32_t)L, S.row_counts.as<uint32_t>(),
+    hipLaunchKernelGGL(sm_crossings_kernel<1>, grid, dim3(kBlock), 0, st, S.lines.as<const double4>(), (uint32_t)L,
+                       n_chunks, S.chunk_counts.as<uint32_t>(), S.masks.as<unsigned long long>(),
                        S.row_off.as<const uint64_t>(), dst.as<double2>());
     LG_HIP(hipGetLastError());
   }
@@ -411,8 +485,7 @@ int lg_string_mod_draw(lg_ctx* ctx, uint32_t n, const lg_string_mod* mods, lg_se
   lg::begin_call(ctx);
   hipStream_t st = ctx->stream;
   lg::SmScratch& S = scratch(ctx);
-  for (lg::DevBuf* b : S.crossings) delete b;
-  S.crossings.clear();
+  size_t n_nested = 0;
 
   // nested StringMods draw over the crossing points of their outer lines (string_mod.rs:153-154)
   std::vector<SmDev> descs;
@@ -424,10 +497,11 @@ int lg_string_mod_draw(lg_ctx* ctx, uint32_t n, const lg_string_mod* mods, lg_se
     SmDev d = make_dev(m, (uint32_t)mrc.size());
     mrc.insert(mrc.end(), m.modulo_colors, m.modulo_colors + m.n_modulo_colors);
     if (outer.nested) {
-      S.crossings.push_back(new lg::DevBuf());
+      if (S.crossings.size() <= n_nested) S.crossings.push_back(new lg::DevBuf());
+      lg::DevBuf& buf = *S.crossings[n_nested++];
       uint64_t np = 0;
-      if ((rc = crossings_device(ctx, outer, *S.crossings.back(), &np))) return rc;
-      d.pts = S.crossings.back()->as<const double2>();
+      if ((rc = crossings_device(ctx, outer, buf, &np))) return rc;
+      d.pts = buf.as<const double2>();
       d.n_points = np;
     }
     // draw_init_points returns nothing when there are no points (string_mod.rs:106-108)
