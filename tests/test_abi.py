@@ -45,6 +45,7 @@ def test_struct_layout_matches_c():
         "lg_params": abi.LgParams, "lg_view": abi.LgView, "lg_stage_time": abi.LgStageTime,
         "lg_tuning": abi.LgTuning, "lg_server_config": abi.LgServerConfig, "lg_request": abi.LgRequest,
         "lg_result": abi.LgResult, "lg_server_stats": abi.LgServerStats,
+        "lg_mod_rem_color": abi.LgModRemColor, "lg_string_mod": abi.LgStringMod,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for cname, cls in fields.items():
