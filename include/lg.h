@@ -242,6 +242,50 @@ int lg_image_device_ptrs(lg_image* img, void** accum, void** rgba16f);
 int lg_image_download_rgba16f(lg_image* img, uint16_t* host, uint64_t cap_halves);
 int lg_image_download_accum(lg_image* img, float* host, uint64_t cap_floats);
 
+/* ---- blend state of an image: the lines pipeline's BlendState (mod.rs:57-73), selectable
+ * in the reference's settings window (gui/settings.rs:57-133). Values follow wgpu's enums in
+ * the order that window lists them. */
+enum lg_blend_factor {
+  LG_BF_ZERO = 0, LG_BF_ONE = 1, LG_BF_SRC = 2, LG_BF_ONE_MINUS_SRC = 3, LG_BF_SRC_ALPHA = 4,
+  LG_BF_ONE_MINUS_SRC_ALPHA = 5, LG_BF_DST = 6, LG_BF_ONE_MINUS_DST = 7, LG_BF_DST_ALPHA = 8,
+  LG_BF_ONE_MINUS_DST_ALPHA = 9, LG_BF_SRC_ALPHA_SATURATED = 10, LG_BF_CONSTANT = 11,
+  LG_BF_ONE_MINUS_CONSTANT = 12
+};
+enum lg_blend_op { LG_BO_MIN = 0, LG_BO_MAX = 1, LG_BO_ADD = 2, LG_BO_SUBTRACT = 3, LG_BO_REVERSE_SUBTRACT = 4 };
+typedef struct lg_blend_component {   /* wgpu::BlendComponent */
+  uint32_t src_factor;                /* lg_blend_factor */
+  uint32_t dst_factor;
+  uint32_t operation;                 /* lg_blend_op */
+} lg_blend_component;
+typedef struct lg_blend {             /* wgpu::BlendState + the blend constant */
+  lg_blend_component color;           /* default: One, One, Add        (mod.rs:66-70) */
+  lg_blend_component alpha;           /* default: SrcAlpha, One, Add   (mod.rs:61-65) */
+  float constant[4];                  /* default 0 (never set by the reference) */
+} lg_blend;
+/* Sets the blend of every later accumulation into img. Supported: the states whose result
+ * does not depend on the order of the fragments nor on the target's value, i.e. dst factor
+ * One with Add or ReverseSubtract and a source factor among Zero, One, Src, OneMinusSrc,
+ * SrcAlpha, OneMinusSrcAlpha, Constant, OneMinusConstant: each sample then adds
+ * +-w * src * factor, which the exact integer tile sums take in any order. The others (Dst*
+ * factors, SrcAlphaSaturated, a dst factor other than One, Subtract, Min, Max) return
+ * LG_E_UNSUPPORTED; out-of-range values LG_E_INVALID. */
+int lg_image_set_blend(lg_image* img, const lg_blend* blend);
+int lg_image_get_blend(const lg_image* img, lg_blend* out);
+
+/* ---- 8-bit encode: Renderer::make_screenshot's pixel conversion (renderer.rs:313-328) ---- */
+enum lg_encode_order {
+  LG_ENCODE_BGRA = 0,  /* the reference's bytes: B, G, R, A (it then saves them as RGBA) */
+  LG_ENCODE_RGBA = 1
+};
+/* Each RGBA16F channel of img's resolved target -> f32 -> powf(f, 1/2.2) * 255 -> u8 with
+ * Rust's saturating `as` (NaN and values <= 0 give 0, >= 255 give 255, else truncation), on the
+ * GPU; host gets width * height * 4 bytes, row 0 at the top (the copy of renderer.rs:271-291
+ * with its row padding removed). The JPEG file write is not part of it. */
+int lg_image_encode_8bit(lg_image* img, uint32_t order, uint8_t* host, uint64_t cap_bytes);
+/* The conversion as a table: table[h] = the u8 of the half with bits h (65536 entries). It is
+ * computed on the device (FP64 pow rounded to FP32) and is what lg_image_encode_8bit applies. */
+int lg_encode_table(lg_ctx* ctx, uint8_t* table);
+
 /* ---- accumulation: LineList draw + One/One blend (shader.wgsl:14-28, mod.rs:57-73) ---- */
 /* Adds every segment into img's FP32 accumulator: rgb += w*rgb, a += w*a*a, with the
  * anti-aliased column/row sampling rule of this build. */

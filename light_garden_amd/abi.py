@@ -134,6 +134,21 @@ LgStringMod._fields_ = [
 ]
 
 
+BF_ZERO, BF_ONE, BF_SRC, BF_ONE_MINUS_SRC, BF_SRC_ALPHA, BF_ONE_MINUS_SRC_ALPHA = 0, 1, 2, 3, 4, 5
+BF_DST, BF_ONE_MINUS_DST, BF_DST_ALPHA, BF_ONE_MINUS_DST_ALPHA, BF_SRC_ALPHA_SATURATED = 6, 7, 8, 9, 10
+BF_CONSTANT, BF_ONE_MINUS_CONSTANT = 11, 12
+BO_MIN, BO_MAX, BO_ADD, BO_SUBTRACT, BO_REVERSE_SUBTRACT = 0, 1, 2, 3, 4
+ENCODE_BGRA, ENCODE_RGBA = 0, 1
+
+
+class LgBlendComponent(C.Structure):
+    _fields_ = [("src_factor", C.c_uint32), ("dst_factor", C.c_uint32), ("operation", C.c_uint32)]
+
+
+class LgBlend(C.Structure):
+    _fields_ = [("color", LgBlendComponent), ("alpha", LgBlendComponent), ("constant", C.c_float * 4)]
+
+
 assert C.sizeof(LgShape) == 136
 assert C.sizeof(LgObject) == 16 + 7 * 136 + 256
 assert C.sizeof(LgLight) == 72
@@ -195,6 +210,10 @@ FUNCTIONS = {
     "lg_string_mod_draw": (C.c_int, [_P, C.c_uint32, C.POINTER(LgStringMod), _P]),
     "lg_string_mod_crossings": (C.c_int, [_P, C.POINTER(LgStringMod), _P, C.c_uint64,
                                           C.POINTER(C.c_uint64)]),
+    "lg_image_set_blend": (C.c_int, [_P, C.POINTER(LgBlend)]),
+    "lg_image_get_blend": (C.c_int, [_P, C.POINTER(LgBlend)]),
+    "lg_image_encode_8bit": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64]),
+    "lg_encode_table": (C.c_int, [_P, _P]),
     "lg_scene_load_ron": (C.c_int, [_P, C.c_char_p, C.c_uint64]),
     "lg_scene_save_ron": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "lg_ron_parse_scene": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(LgObject), C.c_uint32,

@@ -449,16 +449,34 @@ constexpr int kSlotU64 = 4 * kTile * kTile;  // one merge slot: 4 channel planes
 // from a buffer capacity, so it is the same alone or in any batch). A segment deposits at most
 // once into a given pixel, so every partial sum of a pixel is bounded by
 // n * 2^20 * max|cq|; with n <= 2^lb and G + log2(bound) <= 41 - lb that stays below 2^62,
-// and cq below 2^30.
+// and cq below 2^30. bound = max|colour| * max|blend factor| (max(m, m^2) for the default
+// One / SrcAlpha blend).
 __device__ __forceinline__ int scene_fixed_bits(const SceneDesc& D) {
   const double m = (double)D.color_max;
-  const double bound = m > m * m ? m : m * m;
+  const double bound = m * (double)D.factor_max;
   if (!(bound > 0.0) || !(bound < 1e300)) return 30;
   const uint64_t n = scene_segments(D);
   const int lb = n > 1 ? 64 - __clzll((long long)(n - 1)) : 0;  // n <= 2^lb
   const double room = (42 - lb) < 30 ? (double)(42 - lb) : 30.0;
   const int G = (int)floor(room - log2(bound)) - 1;
   return G < -30 ? -30 : (G > 60 ? 60 : G);
+}
+
+// The value one sample of colour channel c adds per unit weight: +-c * factor (BlendCode).
+__device__ __forceinline__ float blend_value(uint32_t code, float c, float a, float k) {
+  float f;
+  switch (code & 7u) {
+    case BC_ZERO: f = 0.0f; break;
+    case BC_ONE: f = 1.0f; break;
+    case BC_SRC: f = c; break;
+    case BC_ONE_MINUS_SRC: f = 1.0f - c; break;
+    case BC_SRC_ALPHA: f = a; break;
+    case BC_ONE_MINUS_SRC_ALPHA: f = 1.0f - a; break;
+    case BC_CONST: f = k; break;
+    default: f = 1.0f - k; break;
+  }
+  const float v = c * f;
+  return (code & BC_NEGATE) ? -v : v;
 }
 
 __device__ __forceinline__ void lds_deposit(unsigned long long* tile, int row, int col, int w,
@@ -525,6 +543,9 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
     float4* accum = gen(D.accum);
     const int G = scene_fixed_bits(D);
     const float cscale = __builtin_ldexpf(1.0f, G);
+    const uint32_t bc = __builtin_amdgcn_readfirstlane(D.blend_code);
+    const uint32_t bc0 = bc & 15u, bc1 = (bc >> 4) & 15u, bc2 = (bc >> 8) & 15u, bc3 = (bc >> 12) & 15u;
+    const float bk0 = D.blend_k[0], bk1 = D.blend_k[1], bk2 = D.blend_k[2], bk3 = D.blend_k[3];
     const double unscale = __builtin_ldexp(1.0, -(G + kWBits));
     for (int k = threadIdx.x; k < 4 * kPlane; k += BLOCK) s_tile[k] = 0ull;
     __syncthreads();
@@ -557,10 +578,10 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
             const int vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
             s_anchor = (float)(u0 * g.inv_dM);
             ds = (float)g.inv_dM;
-            cq0 = fx(ca.x, cscale);
-            cq1 = fx(ca.y, cscale);
-            cq2 = fx(ca.z, cscale);
-            cq3 = fx(ca.w * ca.w, cscale);
+            cq0 = fx(blend_value(bc0, ca.x, ca.w, bk0), cscale);
+            cq1 = fx(blend_value(bc1, ca.y, ca.w, bk1), cscale);
+            cq2 = fx(blend_value(bc2, ca.z, ca.w, bk2), cscale);
+            cq3 = fx(blend_value(bc3, ca.w, ca.w, bk3), cscale);
             const int neg = (cq0 | cq1 | cq2 | cq3) < 0 ? 1 : 0;
             packed = (int)(b - a + 1) | ((int)(a - maj0) << 8) | (g.xmajor << 16) | (vc << 17) | (neg << 18);
           }
@@ -649,10 +670,10 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
             const int wl = kWOne - wq;
             const float sp = __builtin_fmaf((float)q, jds, jsa);
             const float al = __builtin_fmaf(sp, d3, a3);
-            const int c0 = fx(__builtin_fmaf(sp, d0, a0), cscale);
-            const int c1 = fx(__builtin_fmaf(sp, d1, a1), cscale);
-            const int c2 = fx(__builtin_fmaf(sp, d2, a2), cscale);
-            const int c3 = fx(al * al, cscale);
+            const int c0 = fx(blend_value(bc0, __builtin_fmaf(sp, d0, a0), al, bk0), cscale);
+            const int c1 = fx(blend_value(bc1, __builtin_fmaf(sp, d1, a1), al, bk1), cscale);
+            const int c2 = fx(blend_value(bc2, __builtin_fmaf(sp, d2, a2), al, bk2), cscale);
+            const int c3 = fx(blend_value(bc3, al, al, bk3), cscale);
             // 24-bit multiplies (jl >= -1 wherever a deposit is made); i1 = i0 + rstep
             const int i1 = (int)(__umul24((unsigned)(jl + 1), (unsigned)rstep) + __umul24((unsigned)q, (unsigned)qstep));
             const int i0 = i1 - rstep;

@@ -124,6 +124,8 @@ struct lg_ctx {
   std::vector<lg_stage_time> last_times;
   std::unique_ptr<lg::Pipeline, lg::PipelineDeleter> pipe;
   std::unique_ptr<lg::SmScratch, lg::SmScratchDeleter> sm;
+  lg::DevBuf encode_table;  // 65536 u8: the 8-bit encode of every half (built on first use)
+  bool encode_table_ready = false;
 
   hipEvent_t get_event() {
     if (!event_pool.empty()) {
@@ -167,6 +169,9 @@ struct lg_image {
   lg_ctx* ctx = nullptr;
   lg_view view{};
   lg::DevBuf accum, out16;
+  lg_blend blend{{LG_BF_ONE, LG_BF_ONE, LG_BO_ADD}, {LG_BF_SRC_ALPHA, LG_BF_ONE, LG_BO_ADD}, {0.f, 0.f, 0.f, 0.f}};
+  uint32_t blend_code = lg::kBlendDefault;
+  lg::DevBuf enc8;  // 8-bit encode output
 };
 
 namespace lg {
@@ -189,6 +194,8 @@ int check_params(const lg_params* p);
 void ctx_child_gone(lg_ctx* ctx);
 void begin_call(lg_ctx* ctx);
 void end_call(lg_ctx* ctx);
+// max |blend factor| of a BlendCode word over colours with |channel| <= m
+float blend_factor_bound(uint32_t code, float m, const float k[4]);
 
 // Runs `stages` for every entry as one batched frame on ctx's stream and waits for it.
 int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& entries, uint32_t stages);

@@ -199,7 +199,22 @@ struct SceneDesc {
   int32_t tile_base;          // first global tile of this scene
   float color_max;            // bound on |colour channel| of the scene's segments
   uint64_t px_base;           // first pixel of this scene in the batch's pixel range
+  // blend of the image (lg_image_set_blend): per channel (r, g, b, a) 4 bits at 4 * channel,
+  // a BlendCode factor in the low 3 bits and bit 3 set for ReverseSubtract; the constant;
+  // the largest |factor| over |colour| <= color_max
+  uint32_t blend_code;
+  float blend_k[4];
+  float factor_max;
 };
+
+// Source factors of the supported blends, per channel c of the interpolated colour (a = its
+// alpha, k = the blend constant): each sample adds +-w * c * factor.
+enum BlendCode : uint32_t {
+  BC_ZERO = 0, BC_ONE = 1, BC_SRC = 2, BC_ONE_MINUS_SRC = 3, BC_SRC_ALPHA = 4, BC_ONE_MINUS_SRC_ALPHA = 5,
+  BC_CONST = 6, BC_ONE_MINUS_CONST = 7, BC_NEGATE = 8
+};
+// rgb: One; alpha: SrcAlpha (mod.rs:57-73)
+constexpr uint32_t kBlendDefault = BC_ONE | (BC_ONE << 4) | (BC_ONE << 8) | (BC_SRC_ALPHA << 12);
 
 constexpr int kMaxGen = kPathBits;
 

@@ -381,6 +381,26 @@ class Image:
         abi.call("lg_image_download_accum", self._h, out.ctypes.data, out.size)
         return out
 
+    def set_blend(self, color=(abi.BF_ONE, abi.BF_ONE, abi.BO_ADD),
+                  alpha=(abi.BF_SRC_ALPHA, abi.BF_ONE, abi.BO_ADD), constant=(0.0, 0.0, 0.0, 0.0)):
+        """The lines pipeline's BlendState (mod.rs:57-73): (src_factor, dst_factor, operation)
+        per component, abi.BF_* / abi.BO_* values; defaults are the reference's."""
+        b = blend_struct(color, alpha, constant)
+        abi.call("lg_image_set_blend", self._h, C.byref(b))
+
+    def blend(self):
+        b = abi.LgBlend()
+        abi.call("lg_image_get_blend", self._h, C.byref(b))
+        comp = lambda c: (c.src_factor, c.dst_factor, c.operation)
+        return comp(b.color), comp(b.alpha), tuple(b.constant)
+
+    def encode_8bit(self, order: int = abi.ENCODE_BGRA) -> np.ndarray:
+        """Renderer::make_screenshot's pixel conversion (renderer.rs:313-328) on the GPU:
+        (H, W, 4) uint8, B G R A by default (the reference's byte order)."""
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        abi.call("lg_image_encode_8bit", self._h, int(order), out.ctypes.data, out.size)
+        return out
+
     def close(self):
         if self._h:
             abi.load().lg_image_destroy(self._h)
@@ -391,6 +411,21 @@ class Image:
             self.close()
         except Exception:
             pass
+
+
+def blend_struct(color, alpha, constant=(0.0, 0.0, 0.0, 0.0)) -> abi.LgBlend:
+    b = abi.LgBlend()
+    b.color.src_factor, b.color.dst_factor, b.color.operation = (int(v) for v in color)
+    b.alpha.src_factor, b.alpha.dst_factor, b.alpha.operation = (int(v) for v in alpha)
+    b.constant[:] = [float(v) for v in constant]
+    return b
+
+
+def encode_table(ctx: Context) -> np.ndarray:
+    """The device's 8-bit encode of every half value (lg_encode_table): uint8[65536]."""
+    out = np.empty(65536, np.uint8)
+    abi.call("lg_encode_table", ctx.handle, out.ctypes.data)
+    return out
 
 
 # ---- Tracer (tracer.rs) ------------------------------------------------------------------------

@@ -874,6 +874,17 @@ or_segs* or_trace(const lg_object* objs, uint32_t n_obj, const lg_light* lights,
  *   weight 1 - f and pixel (i, j + 1) weight f. y-major is the same with axes swapped.
  *   A deposit of weight w adds w * rgb to RGB and w * a * a to alpha.
  */
+/*
+ * Blend (mod.rs:57-73, gui/settings.rs:57-133): with dst factor One, the target after all
+ * fragments is clear + sum(sign * src * Fs) whatever their order; that is the set this
+ * restatement (and the library) takes. Fs per channel c of the interpolated colour, a its
+ * alpha, k the blend constant: Zero 0, One 1, Src c (for the alpha channel: a), OneMinusSrc
+ * 1 - c, SrcAlpha a, OneMinusSrcAlpha 1 - a, Constant k, OneMinusConstant 1 - k; sign -1 for
+ * ReverseSubtract. The default (rgb One, alpha SrcAlpha, Add) is rgb += w c, a += w a a.
+ */
+enum { BF_ZERO, BF_ONE, BF_SRC, BF_ONE_MINUS_SRC, BF_SRC_ALPHA, BF_ONE_MINUS_SRC_ALPHA, BF_CONST,
+       BF_ONE_MINUS_CONST };
+
 typedef struct acc_job {
   const void* pos;
   int pos_is_f32;
@@ -884,7 +895,36 @@ typedef struct acc_job {
   void* accum;
   int accum_is_f32;
   int band_lo, band_hi; /* rows [lo, hi) owned by this thread */
+  int factor[4];        /* BF_* per channel */
+  int negate[4];
+  float k[4];
 } acc_job;
+
+static inline double blend_factor_d(int f, double c, double a, double k) {
+  switch (f) {
+    case BF_ZERO: return 0.0;
+    case BF_ONE: return 1.0;
+    case BF_SRC: return c;
+    case BF_ONE_MINUS_SRC: return 1.0 - c;
+    case BF_SRC_ALPHA: return a;
+    case BF_ONE_MINUS_SRC_ALPHA: return 1.0 - a;
+    case BF_CONST: return k;
+    default: return 1.0 - k;
+  }
+}
+
+static inline float blend_factor_f(int f, float c, float a, float k) {
+  switch (f) {
+    case BF_ZERO: return 0.0f;
+    case BF_ONE: return 1.0f;
+    case BF_SRC: return c;
+    case BF_ONE_MINUS_SRC: return 1.0f - c;
+    case BF_SRC_ALPHA: return a;
+    case BF_ONE_MINUS_SRC_ALPHA: return 1.0f - a;
+    case BF_CONST: return k;
+    default: return 1.0f - k;
+  }
+}
 
 static inline void acc_deposit(acc_job* J, int px, int py, double w, const double col[4]) {
   uint32_t W = J->view->width;
@@ -893,16 +933,17 @@ static inline void acc_deposit(acc_job* J, int px, int py, double w, const doubl
   if (J->accum_is_f32) {
     float* a = (float*)J->accum + idx;
     float wf = (float)w;
-    a[0] += wf * (float)col[0];
-    a[1] += wf * (float)col[1];
-    a[2] += wf * (float)col[2];
-    a[3] += wf * (float)col[3] * (float)col[3];
+    const float al = (float)col[3];
+    for (int c = 0; c < 4; c++) {
+      const float v = wf * (float)col[c] * blend_factor_f(J->factor[c], (float)col[c], al, J->k[c]);
+      a[c] += J->negate[c] ? -v : v;
+    }
   } else {
     double* a = (double*)J->accum + idx;
-    a[0] += w * col[0];
-    a[1] += w * col[1];
-    a[2] += w * col[2];
-    a[3] += w * col[3] * col[3];
+    for (int c = 0; c < 4; c++) {
+      const double v = w * col[c] * blend_factor_d(J->factor[c], col[c], col[3], (double)J->k[c]);
+      a[c] += J->negate[c] ? -v : v;
+    }
   }
 }
 
@@ -1008,9 +1049,44 @@ static void* acc_worker(void* arg) {
   return NULL;
 }
 
+/* lg_blend -> per-channel factors; 0 when unsupported (see the block comment above). */
+static int blend_factors(const lg_blend* b, int factor[4], int negate[4], float k[4]) {
+  const lg_blend_component* comp[2] = {&b->color, &b->alpha};
+  for (int i = 0; i < 2; i++) {
+    const lg_blend_component* x = comp[i];
+    if (x->dst_factor != LG_BF_ONE) return 0;
+    if (x->operation != LG_BO_ADD && x->operation != LG_BO_REVERSE_SUBTRACT) return 0;
+    int f;
+    switch (x->src_factor) {
+      case LG_BF_ZERO: f = BF_ZERO; break;
+      case LG_BF_ONE: f = BF_ONE; break;
+      case LG_BF_SRC: f = i ? BF_SRC_ALPHA : BF_SRC; break;
+      case LG_BF_ONE_MINUS_SRC: f = i ? BF_ONE_MINUS_SRC_ALPHA : BF_ONE_MINUS_SRC; break;
+      case LG_BF_SRC_ALPHA: f = BF_SRC_ALPHA; break;
+      case LG_BF_ONE_MINUS_SRC_ALPHA: f = BF_ONE_MINUS_SRC_ALPHA; break;
+      case LG_BF_CONSTANT: f = BF_CONST; break;
+      case LG_BF_ONE_MINUS_CONSTANT: f = BF_ONE_MINUS_CONST; break;
+      default: return 0;
+    }
+    const int lo = i ? 3 : 0, hi = i ? 4 : 3;
+    for (int c = lo; c < hi; c++) {
+      factor[c] = f;
+      negate[c] = x->operation == LG_BO_REVERSE_SUBTRACT;
+    }
+  }
+  for (int c = 0; c < 4; c++) k[c] = b->constant[c];
+  return 1;
+}
+
 static void accumulate_generic(const void* pos, int pos_f32, const float* c0, const float* c1,
                                uint64_t n, const lg_view* view, void* accum, int acc_f32,
-                               int n_threads) {
+                               int n_threads, const lg_blend* blend) {
+  static const lg_blend kDefault = {{LG_BF_ONE, LG_BF_ONE, LG_BO_ADD},
+                                    {LG_BF_SRC_ALPHA, LG_BF_ONE, LG_BO_ADD},
+                                    {0.f, 0.f, 0.f, 0.f}};
+  int factor[4], negate[4];
+  float k[4];
+  if (!blend_factors(blend ? blend : &kDefault, factor, negate, k)) return;
   if (n_threads < 1) n_threads = 1;
   int H = (int)view->height;
   if (n_threads > H) n_threads = H;
@@ -1028,6 +1104,11 @@ static void accumulate_generic(const void* pos, int pos_f32, const float* c0, co
     J->accum_is_f32 = acc_f32;
     J->band_lo = (int)((long)H * t / n_threads);
     J->band_hi = (int)((long)H * (t + 1) / n_threads);
+    for (int c = 0; c < 4; c++) {
+      J->factor[c] = factor[c];
+      J->negate[c] = negate[c];
+      J->k[c] = k[c];
+    }
   }
   if (n_threads == 1) {
     acc_worker(&jobs[0]);
@@ -1041,12 +1122,44 @@ static void accumulate_generic(const void* pos, int pos_f32, const float* c0, co
 
 void or_accumulate_f64(const double* pos, const float* color0, const float* color1, uint64_t n,
                        const lg_view* view, double* accum, int n_threads) {
-  accumulate_generic(pos, 0, color0, color1, n, view, accum, 0, n_threads);
+  accumulate_generic(pos, 0, color0, color1, n, view, accum, 0, n_threads, NULL);
 }
 
 void or_accumulate_f32(const float* pos, const float* color0, const float* color1, uint64_t n,
                        const lg_view* view, float* accum, int n_threads) {
-  accumulate_generic(pos, 1, color0, color1, n, view, accum, 1, n_threads);
+  accumulate_generic(pos, 1, color0, color1, n, view, accum, 1, n_threads, NULL);
+}
+
+int or_accumulate_blend_f64(const double* pos, const float* color0, const float* color1, uint64_t n,
+                            const lg_view* view, double* accum, int n_threads, const lg_blend* blend) {
+  int factor[4], negate[4];
+  float k[4];
+  if (!blend_factors(blend, factor, negate, k)) return 0;
+  accumulate_generic(pos, 0, color0, color1, n, view, accum, 0, n_threads, blend);
+  return 1;
+}
+
+/* renderer.rs:325-328: f16 -> f32, powf(f, 1/2.2) * 255, Rust's saturating `as u8`. */
+uint8_t or_encode_value(uint16_t h) {
+  const float f = (float)or_double_from_half(h);
+  const float v = powf(f, 1.0f / 2.2f) * 255.0f;
+  if (!(v > 0.0f)) return 0;
+  if (v >= 255.0f) return 255;
+  return (uint8_t)v;
+}
+
+/* renderer.rs:313-323 (BGRA: the bytes B, G, R, A) or RGBA, per pixel. */
+void or_encode_8bit(const uint16_t* rgba16f, uint64_t n_pixels, int order, uint8_t* out) {
+  for (uint64_t i = 0; i < n_pixels; i++) {
+    const uint8_t r = or_encode_value(rgba16f[4 * i]), g = or_encode_value(rgba16f[4 * i + 1]);
+    const uint8_t b = or_encode_value(rgba16f[4 * i + 2]), a = or_encode_value(rgba16f[4 * i + 3]);
+    uint8_t* o = out + 4 * i;
+    if (order == LG_ENCODE_RGBA) {
+      o[0] = r; o[1] = g; o[2] = b; o[3] = a;
+    } else {
+      o[0] = b; o[1] = g; o[2] = r; o[3] = a;
+    }
+  }
 }
 
 uint64_t or_sample_count(const double* pos, uint64_t n, const lg_view* view) {

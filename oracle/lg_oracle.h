@@ -84,6 +84,14 @@ void or_accumulate_f64(const double* pos, const float* color0, const float* colo
                        uint64_t n, const lg_view* view, double* accum, int n_threads);
 void or_accumulate_f32(const float* pos, const float* color0, const float* color1,
                        uint64_t n, const lg_view* view, float* accum, int n_threads);
+/* The same with a blend state (dst factor One, Add/ReverseSubtract, a source factor that
+ * reads neither the target nor the fragment order); returns 0 and does nothing otherwise. */
+int or_accumulate_blend_f64(const double* pos, const float* color0, const float* color1,
+                            uint64_t n, const lg_view* view, double* accum, int n_threads,
+                            const lg_blend* blend);
+/* Screenshot conversion (renderer.rs:313-328): one half -> u8, and whole images. */
+uint8_t or_encode_value(uint16_t h);
+void or_encode_8bit(const uint16_t* rgba16f, uint64_t n_pixels, int order, uint8_t* out);
 /* Fragment count of the rule: number of column/row samples. */
 uint64_t or_sample_count(const double* pos, uint64_t n, const lg_view* view);
 /* FP32/FP64 accumulator -> RGBA16F bits, alpha + 1. */

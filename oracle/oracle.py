@@ -68,6 +68,9 @@ def load():
         "or_line_crossings": (C.c_uint64, [P, C.c_uint64, P, C.c_uint64]),
         "or_sm_crossings": (C.c_uint64, [P, P, C.c_uint64]),
         "or_sm_draw": (C.c_uint64, [P, P, P, P]),
+        "or_accumulate_blend_f64": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_int, P]),
+        "or_encode_value": (C.c_uint8, [C.c_uint16]),
+        "or_encode_8bit": (None, [P, C.c_uint64, C.c_int, P]),
         "or_now_seconds": (C.c_double, []),
     }
     for name, (res, args) in sig.items():
@@ -172,6 +175,29 @@ def accumulate_f32(pos, color0, color1, view, threads=1, acc=None):
     load().or_accumulate_f32(pos.ctypes.data, c0.ctypes.data, None if c1 is None else c1.ctypes.data,
                              pos.shape[0], C.byref(view), acc.ctypes.data, int(threads))
     return acc
+
+
+def accumulate_blend_f64(pos, color0, color1, view, blend, threads=1):
+    """accumulate_f64 with an abi.LgBlend; None when the blend is not order-free."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 4)
+    c0 = np.ascontiguousarray(color0, np.float32).reshape(-1, 4)
+    c1 = None if color1 is None else np.ascontiguousarray(color1, np.float32).reshape(-1, 4)
+    acc = np.zeros((view.height, view.width, 4), np.float64)
+    ok = load().or_accumulate_blend_f64(pos.ctypes.data, c0.ctypes.data, None if c1 is None else c1.ctypes.data,
+                                        pos.shape[0], C.byref(view), acc.ctypes.data, int(threads), C.byref(blend))
+    return acc if ok else None
+
+
+def encode_value(h):
+    return load().or_encode_value(int(h))
+
+
+def encode_8bit(rgba16f, order=0):
+    """(H, W, 4) uint16 half bits -> (H, W, 4) uint8 (order 0: B G R A, 1: R G B A)."""
+    src = np.ascontiguousarray(rgba16f, np.uint16)
+    out = np.empty(src.shape, np.uint8)
+    load().or_encode_8bit(src.ctypes.data, src.size // 4, int(order), out.ctypes.data)
+    return out
 
 
 def sample_count(pos, view):

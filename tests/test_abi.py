@@ -46,6 +46,7 @@ def test_struct_layout_matches_c():
         "lg_tuning": abi.LgTuning, "lg_server_config": abi.LgServerConfig, "lg_request": abi.LgRequest,
         "lg_result": abi.LgResult, "lg_server_stats": abi.LgServerStats,
         "lg_mod_rem_color": abi.LgModRemColor, "lg_string_mod": abi.LgStringMod,
+        "lg_blend_component": abi.LgBlendComponent, "lg_blend": abi.LgBlend,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for cname, cls in fields.items():
