@@ -91,8 +91,10 @@ def cpu_leg(scene, img, result, stride):
     view = scene.view()
     times = []
     nseg_cpu = 0
-    for rep in range(4):  # 1 warm-up + 3 timed
-        acc = np.zeros((scene.height, scene.width, 4), np.float32)
+    acc = np.zeros((scene.height, scene.width, 4), np.float32)
+    budget_s, spent, rep = 8.0, 0.0, 0
+    while rep < 4 or (spent < budget_s and rep < 2000):  # 1 warm-up, then ~8 s of timed frames
+        acc.fill(0.0)
         t1 = time.perf_counter()
         pos, col, key = orc.trace(objs, len(scene.objects), lights, len(scene.lights), scene.params(),
                                   ray_stride=stride, threads=threads)
@@ -101,14 +103,18 @@ def cpu_leg(scene, img, result, stride):
         nseg_cpu = len(key)
         if rep:
             times.append(dt)
+            spent += dt
+        rep += 1
     med = sorted(times)[len(times) // 2]
     result["cpu_baseline"] = {
         "value": nseg_cpu / med, "unit": "ray-segments/s", "cores": threads, "kind": "port",
         "sample": "every %d-th primary ray of the same scene (%d segments): FP64 trace (chunks of "
-                  "100 rays, %d threads) + FP32 accumulation, median of 3 after 1 warm-up, %.2f s each"
-                  % (stride, nseg_cpu, threads, med),
+                  "100 rays, %d threads) + FP32 accumulation; median of %d frames (%.1f s) after 1 "
+                  "warm-up, %.3f s each" % (stride, nseg_cpu, threads, len(times), spent, med),
     }
     result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    if "graph_replay" in result:
+        result["speedup_vs_cpu_graph_replay"] = result["graph_replay"]["value"] / result["cpu_baseline"]["value"]
 
 
 def stage_table(stage_sum, steps):

@@ -356,6 +356,9 @@ typedef struct lg_tuning {
   uint32_t min_piece;      /* smallest tile work item in segment-tile pairs, >= 64; default 4096 */
   uint32_t max_piece;      /* largest tile work item, >= min_piece; default 8192 */
   uint32_t trace_waves;    /* trace workgroups per compute unit, [1, 16]; default 4 */
+  uint32_t bvh_min_objects; /* scenes with at least this many objects cull through their
+                               bounding-volume hierarchy, others test every object;
+                               0xffffffff: never; default 16 */
 } lg_tuning;
 int lg_ctx_get_tuning(const lg_ctx* ctx, lg_tuning* out);
 int lg_ctx_set_tuning(lg_ctx* ctx, const lg_tuning* tuning);

@@ -291,6 +291,146 @@ static int upload(DevBuf& b, const U* data, size_t n, hipStream_t st) {
   return LG_OK;
 }
 
+// ---- object bounding volumes -------------------------------------------------------------
+struct Box2 {
+  double lo[2], hi[2];
+  void add(double x, double y) {
+    lo[0] = std::min(lo[0], x);
+    lo[1] = std::min(lo[1], y);
+    hi[0] = std::max(hi[0], x);
+    hi[1] = std::max(hi[1], y);
+  }
+  void add(const Box2& b) {
+    add(b.lo[0], b.lo[1]);
+    add(b.hi[0], b.hi[1]);
+  }
+};
+
+static Box2 empty_box() { return Box2{{INFINITY, INFINITY}, {-INFINITY, -INFINITY}}; }
+
+// World box of one primitive record (REC_* parameterisation).
+static Box2 rec_box(const Rec<double>& R, const std::vector<double>& verts) {
+  Box2 b = empty_box();
+  const double* p = R.p;
+  switch (R.kind) {
+    case REC_SEG: b.add(p[0], p[1]); b.add(p[0] + p[2], p[1] + p[3]); break;
+    case REC_BEZ: b.add(p[8], p[9]); b.add(p[10], p[11]); break;
+    case REC_CIRCLE: b.add(p[0] - p[2], p[1] - p[2]); b.add(p[0] + p[2], p[1] + p[2]); break;
+    case REC_RECT: {
+      // world = c + x u + y v, u = (p2, p3), v = (p4, p5), |x| <= hw, |y| <= hh
+      const double hx = std::fabs(p[2]) * p[6] + std::fabs(p[4]) * p[7];
+      const double hy = std::fabs(p[3]) * p[6] + std::fabs(p[5]) * p[7];
+      b.add(p[0] - hx, p[1] - hy);
+      b.add(p[0] + hx, p[1] + hy);
+      break;
+    }
+    case REC_ELLIPSE: {
+      const double hx = std::hypot(p[6] * p[2], p[7] * p[4]);
+      const double hy = std::hypot(p[6] * p[3], p[7] * p[5]);
+      b.add(p[0] - hx, p[1] - hy);
+      b.add(p[0] + hx, p[1] + hy);
+      break;
+    }
+    case REC_POLY:
+      for (int i = 0; i < R.op; i++) {
+        const double x = verts[2 * (R.aux + i)], y = verts[2 * (R.aux + i) + 1];
+        b.add(p[0] + p[2] * x + p[4] * y, p[1] + p[3] * x + p[5] * y);
+      }
+      break;
+    default: break;
+  }
+  return b;
+}
+
+static float round_down(double v) {
+  float f = (float)v;
+  return (double)f > v ? std::nextafter(f, -INFINITY) : f;
+}
+static float round_up(double v) {
+  float f = (float)v;
+  return (double)f < v ? std::nextafter(f, INFINITY) : f;
+}
+
+static uint32_t spread_bits(uint32_t v) {  // 16 bits -> even bit positions
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// Preorder node of objects order[lo, hi) (median split); returns its index.
+static int bvh_build(std::vector<BvhNode>& nodes, const std::vector<int>& order, const std::vector<Box2>& boxes,
+                     const std::vector<int32_t>& obj_rec, int lo, int hi) {
+  const int idx = (int)nodes.size();
+  nodes.push_back(BvhNode{});
+  Box2 b = empty_box();
+  if (hi - lo <= kBvhLeaf) {
+    BvhNode& N = nodes[idx];
+    N.count = hi - lo;
+    for (int k = 0; k < kBvhLeaf; k++) N.top[k] = k < hi - lo ? obj_rec[order[lo + k]] : -1;
+    for (int k = lo; k < hi; k++) b.add(boxes[order[k]]);
+  } else {
+    const int mid = (lo + hi) / 2;
+    bvh_build(nodes, order, boxes, obj_rec, lo, mid);
+    bvh_build(nodes, order, boxes, obj_rec, mid, hi);
+    for (int k = lo; k < hi; k++) b.add(boxes[order[k]]);
+    nodes[idx].count = 0;
+    for (int k = 0; k < kBvhLeaf; k++) nodes[idx].top[k] = -1;
+  }
+  BvhNode& N = nodes[idx];
+  N.lo[0] = round_down(b.lo[0]);
+  N.lo[1] = round_down(b.lo[1]);
+  N.hi[0] = round_up(b.hi[0]);
+  N.hi[1] = round_up(b.hi[1]);
+  N.skip = (int)nodes.size();
+  return idx;
+}
+
+// The scene's BVH: object boxes padded by 1e-5 (1 + |coordinates|), far more than the FP32
+// culling pass's error, leaves of kBvhLeaf objects in Morton order of their box centres.
+static std::vector<BvhNode> build_bvh(const std::vector<Rec<double>>& recs, const std::vector<double>& verts,
+                                      const std::vector<int32_t>& obj_rec) {
+  const int n = (int)obj_rec.size();
+  std::vector<Box2> boxes(n);
+  Box2 all = empty_box();
+  for (int o = 0; o < n; o++) {
+    const int top = obj_rec[o];
+    Box2 b = empty_box();
+    if (recs[top].kind == REC_LOGIC) {
+      b.add(rec_box(recs[top + 1], verts));
+      b.add(rec_box(recs[top + 2], verts));
+    } else {
+      b = rec_box(recs[top], verts);
+    }
+    const double mag = std::max(std::max(std::fabs(b.lo[0]), std::fabs(b.lo[1])),
+                                std::max(std::fabs(b.hi[0]), std::fabs(b.hi[1])));
+    const double pad = 1e-5 * (1.0 + mag);
+    b.lo[0] -= pad;
+    b.lo[1] -= pad;
+    b.hi[0] += pad;
+    b.hi[1] += pad;
+    boxes[o] = b;
+    all.add(b);
+  }
+  std::vector<uint32_t> code(n);
+  const double sx = all.hi[0] > all.lo[0] ? 65535.0 / (all.hi[0] - all.lo[0]) : 0.0;
+  const double sy = all.hi[1] > all.lo[1] ? 65535.0 / (all.hi[1] - all.lo[1]) : 0.0;
+  for (int o = 0; o < n; o++) {
+    const double cx = 0.5 * (boxes[o].lo[0] + boxes[o].hi[0]), cy = 0.5 * (boxes[o].lo[1] + boxes[o].hi[1]);
+    const uint32_t qx = (uint32_t)std::min(65535.0, std::max(0.0, (cx - all.lo[0]) * sx));
+    const uint32_t qy = (uint32_t)std::min(65535.0, std::max(0.0, (cy - all.lo[1]) * sy));
+    code[o] = spread_bits(qx) | (spread_bits(qy) << 1);
+  }
+  std::vector<int> order(n);
+  for (int o = 0; o < n; o++) order[o] = o;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return code[a] < code[b]; });
+  std::vector<BvhNode> nodes;
+  if (n > 0) bvh_build(nodes, order, boxes, obj_rec, 0, n);
+  return nodes;
+}
+
 // Uploads on `st`, the stream of the frame that will read the scene.
 int compile_scene(lg_scene* sc, hipStream_t st) {
   if (!sc->dirty) return LG_OK;
@@ -329,6 +469,9 @@ int compile_scene(lg_scene* sc, hipStream_t st) {
   if ((rc = upload(sc->obj_rec, obj_rec.data(), obj_rec.size(), st))) return rc;
   if ((rc = upload(sc->verts32, verts32.data(), verts32.size(), st))) return rc;
   if ((rc = upload(sc->verts64, verts.data(), verts.size(), st))) return rc;
+  const std::vector<BvhNode> bvh = build_bvh(recs, verts, obj_rec);
+  if ((rc = upload(sc->bvh_dev, bvh.data(), bvh.size(), st))) return rc;
+  sc->n_bvh = (int)bvh.size();
   // lights
   if (sc->lights.size() > (size_t)kMaxLights) return fail(LG_E_UNSUPPORTED, "scene: too many lights");
   std::vector<DevLight> dl;

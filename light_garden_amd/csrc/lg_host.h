@@ -71,11 +71,11 @@ int fail(int code, const std::string& msg);
 
 // ---- kernel launchers (lg_trace.hip, lg_accum.hip) ---------------------------------------------
 template <typename T>
-size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts);
+size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts, int n_nodes);
 template <typename T>
 hipError_t launch_trace(const SceneDesc* descs, int n_scenes, uint32_t gen, const uint32_t* item_pre,
-                        BatchCtl* ctl, uint32_t rpi, int cap_recs, int cap_objs, int cap_verts, int grid,
-                        hipStream_t stream);
+                        BatchCtl* ctl, uint32_t rpi, int cap_recs, int cap_objs, int cap_verts, int cap_nodes,
+                        uint32_t* item_pre_next, int grid, hipStream_t stream);
 hipError_t launch_build_trace_items(const SceneDesc* descs, int n_scenes, uint32_t gen, uint32_t rpi,
                                     uint32_t* item_pre, BatchCtl* ctl, hipStream_t stream);
 int trace_block();
@@ -118,7 +118,7 @@ struct lg_ctx {
   bool closing = false;  // lg_ctx_destroy was called while children were alive
   bool profiling = false;
   bool use_graphs = true;
-  lg_tuning tune{256u, 4096u, 8192u, 4u};
+  lg_tuning tune{256u, 4096u, 8192u, 4u, 16u};
   std::vector<lg::StageRec> pending;  // events of the current call
   std::vector<hipEvent_t> event_pool;
   std::vector<lg_stage_time> last_times;
@@ -148,7 +148,8 @@ struct lg_scene {
   std::vector<int32_t> h_obj_rec;
   std::vector<double> h_verts64;
   int n_recs = 0, n_objs = 0, n_verts = 0;
-  lg::DevBuf recs32, recs64, obj_n64, obj_flags, obj_rec, verts32, verts64, lights_dev;
+  lg::DevBuf recs32, recs64, obj_n64, obj_flags, obj_rec, verts32, verts64, lights_dev, bvh_dev;
+  int n_bvh = 0;              // BVH nodes built for the scene (0: none)
   lg::DevBuf q0, q1;          // ray queues (QRay<double>)
   uint64_t q_cap = 0;         // rays per queue
   uint64_t seg_hint = 0;      // high-water segment count

@@ -103,6 +103,19 @@ __host__ __device__ inline bool contains_object(const Rec<T>* recs, const T* ver
 // Per-object flags.
 enum ObjFlags : uint32_t { OBJ_MATERIAL = 1u, OBJ_CLOSED = 2u };
 
+// Bounding-volume hierarchy over a scene's objects, nodes in preorder: an internal node's
+// first child follows it, `skip` is the node after its subtree. A leaf (count > 0) names up
+// to kBvhLeaf objects by their first record. Boxes are FP32, padded so that every hit the
+// FP32 culling pass can report lies inside its object's box.
+constexpr int kBvhLeaf = 4;
+struct alignas(16) BvhNode {
+  float lo[2], hi[2];
+  int32_t skip;
+  int32_t count;
+  int32_t top[kBvhLeaf];
+};
+static_assert(sizeof(BvhNode) == 48, "BvhNode is three float4");
+
 constexpr int kMaxRecords = 1024;      // records resident in LDS per trace workgroup
 constexpr int kMaxObjects = 1024;
 constexpr int kMaxPolyVertices = 1024; // vertices of all polygons in a scene
@@ -171,6 +184,9 @@ struct SceneDesc {
   const LG_G void* verts;          // polygon vertices in the culling precision
   const LG_G double* verts64;
   const LG_G DevLight* lights;
+  const LG_G BvhNode* bvh;         // NULL: the culling pass walks every record
+  int32_t n_bvh;                   // nodes
+  int32_t _pad_bvh;
   int32_t n_recs, n_objs, n_verts, n_lights;
   uint64_t n_primary;
   uint32_t max_bounce;
@@ -222,6 +238,7 @@ constexpr int kMaxGen = kPathBits;
 struct BatchCtl {
   uint32_t n_items_gen[kMaxGen];   // trace items per generation
   uint32_t item_cursor[kMaxGen];
+  uint32_t gen_done[kMaxGen];      // trace workgroups finished per generation
   uint32_t n_chunks;
   uint32_t chunk;                  // segments per bin chunk
   uint32_t n_counts;               // entries of the [scene][tile][chunk] count matrix

@@ -75,11 +75,16 @@ void PipelineDeleter::operator()(Pipeline* p) const { delete p; }
 
 void pipeline_destroy(lg_ctx* ctx) { ctx->pipe.reset(); }
 
+// Whether a scene's culling pass walks its BVH (FP32 traces of scenes with enough objects).
+static bool scene_uses_bvh(const lg_ctx* ctx, const lg_scene* sc, bool fp64) {
+  return !fp64 && sc->n_bvh > 0 && (uint32_t)sc->n_objs >= ctx->tune.bvh_min_objects;
+}
+
 struct Layout {
   int n = 0;
   bool fp64 = false;
   uint32_t max_bounce = 0;
-  int cap_recs = 0, cap_objs = 0, cap_verts = 0;
+  int cap_recs = 0, cap_objs = 0, cap_verts = 0, cap_nodes = 0;
   int n_tiles = 0, max_tiles = 0;
   uint64_t n_counts = 0, n_px = 0;  // n_counts: capacity of the per-frame bin count matrix
   uint32_t bin_target = 0;           // bin chunks per frame
@@ -118,15 +123,18 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const
   if (stages & ST_TRACE) {
     Scope s(ctx, profile, L.fp64 ? "trace_fp64" : "trace", L.max_bounce);
     const int grid = ctx->n_cu * (int)ctx->tune.trace_waves;
+    // generation 0's work list by its own launch; each later one by the last workgroup of the
+    // previous generation's trace
+    LG_HIP(launch_build_trace_items(d_descs, n, 0, ctx->tune.rays_per_item, P.item_pre.as<uint32_t>(), d_ctl, st));
     for (uint32_t g = 0; g < L.max_bounce; g++) {
       uint32_t* pre = P.item_pre.as<uint32_t>() + (size_t)g * (n + 1);
-      LG_HIP(launch_build_trace_items(d_descs, n, g, ctx->tune.rays_per_item, pre, d_ctl, st));
+      uint32_t* next = g + 1 < L.max_bounce ? pre + (n + 1) : nullptr;
       if (L.fp64)
         LG_HIP(launch_trace<double>(d_descs, n, g, pre, d_ctl, ctx->tune.rays_per_item, L.cap_recs, L.cap_objs,
-                                    L.cap_verts, grid, st));
+                                    L.cap_verts, 0, next, grid, st));
       else
         LG_HIP(launch_trace<float>(d_descs, n, g, pre, d_ctl, ctx->tune.rays_per_item, L.cap_recs, L.cap_objs,
-                                   L.cap_verts, grid, st));
+                                   L.cap_verts, L.cap_nodes, next, grid, st));
     }
   }
   if (stages & ST_ACCUM) {
@@ -229,6 +237,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
   // ---- capacities ----
   L.max_bounce = 0;
   L.cap_recs = L.cap_objs = L.cap_verts = 1;
+  L.cap_nodes = 0;
   for (const Entry& e : E) {
     if (trace) {
       lg_scene* sc = e.scene;
@@ -245,11 +254,16 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
       L.cap_recs = std::max(L.cap_recs, sc->n_recs);
       L.cap_objs = std::max(L.cap_objs, sc->n_objs);
       L.cap_verts = std::max(L.cap_verts, sc->n_verts);
+      if (scene_uses_bvh(ctx, sc, L.fp64)) L.cap_nodes = std::max(L.cap_nodes, sc->n_bvh);
     }
   }
   if (trace) {
-    size_t lds = L.fp64 ? trace_lds_bytes<double>(L.cap_recs, L.cap_objs, L.cap_verts)
-                        : trace_lds_bytes<float>(L.cap_recs, L.cap_objs, L.cap_verts);
+    size_t lds = L.fp64 ? trace_lds_bytes<double>(L.cap_recs, L.cap_objs, L.cap_verts, 0)
+                        : trace_lds_bytes<float>(L.cap_recs, L.cap_objs, L.cap_verts, L.cap_nodes);
+    if (lds > 64 * 1024 && L.cap_nodes) {
+      L.cap_nodes = 0;  // no room for the hierarchies next to the records: every scene walks its records
+      lds = trace_lds_bytes<float>(L.cap_recs, L.cap_objs, L.cap_verts, 0);
+    }
     if (lds > 64 * 1024) return fail(LG_E_UNSUPPORTED, "scene too large for the LDS-resident trace");
   }
   // ---- layout ----
@@ -341,6 +355,9 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
       D.verts = gbl(L.fp64 ? sc->verts64.p : sc->verts32.p);
       D.verts64 = gbl(sc->verts64.as<const double>());
       D.lights = gbl(sc->lights_dev.as<const DevLight>());
+      const bool bvh = L.cap_nodes > 0 && scene_uses_bvh(ctx, sc, L.fp64);
+      D.bvh = gbl(bvh ? sc->bvh_dev.as<const BvhNode>() : nullptr);
+      D.n_bvh = bvh ? sc->n_bvh : 0;
       D.n_recs = sc->n_recs;
       D.n_objs = sc->n_objs;
       D.n_verts = sc->n_verts;
@@ -391,7 +408,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
                                  (uint64_t)L.cap_objs, (uint64_t)L.cap_verts, (uint64_t)L.n_tiles,
                                  (uint64_t)L.max_tiles, L.n_counts, L.n_px, L.bin_target, P.pairs_cap, P.items_cap,
                                  P.split_cap, ctx->tune.rays_per_item, ctx->tune.min_piece, ctx->tune.max_piece,
-                                 ctx->tune.trace_waves};
+                                 ctx->tune.trace_waves, ctx->tune.bvh_min_objects, (uint64_t)L.cap_nodes};
   for (DevBuf* b : {&P.descs, &P.counters, &P.ctl, &P.item_pre, &P.chunk_pre, &P.cbase, &P.counts, &P.offsets,
                     &P.scan_scratch, &P.tile_scene, &P.tile_start, &P.pieces, &P.pairs, &P.items,
                     &P.px_pre, &P.tile_split, &P.split_tiles, &P.split_fx})

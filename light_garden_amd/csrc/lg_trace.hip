@@ -368,8 +368,11 @@ __device__ __forceinline__ bool bezier_nearest(const RayT<T>& r, const T* a, T& 
         }
         const T dfm = ((T)3 * A * x + (T)2 * B) * x + C;
         T xn = x - fm / dfm;
+        // converged: the step no longer moves x (tested before the safeguard, which would
+        // otherwise replace the converged x, now an end of the bracket, by the midpoint)
+        if (xn == x) break;
         if (!(xn > lo && xn < hi)) xn = (T)0.5 * (lo + hi);
-        if (xn <= lo || xn >= hi || xn == x) break;  // the bracket cannot shrink further
+        if (xn <= lo || xn >= hi) break;  // the bracket cannot shrink further
         x = xn;
       }
       root = lo == hi ? lo : x;
@@ -440,11 +443,14 @@ __device__ __forceinline__ void nearest_init(Nearest<T>& nb) {
   nb.top2 = -1;
 }
 
-// Candidates arrive in object order; strict < keeps the first minimum (tracer.rs:404,417).
+// The first minimum in object order (tracer.rs:404,417): a smaller t wins, and an equal t
+// from an object earlier in the scene. With objects offered in scene order this is the strict
+// <; with the hierarchy's order it gives the same winner, and t2 is the smallest t of the
+// other objects either way.
 template <typename T>
 __device__ __forceinline__ void consider(Nearest<T>& nb, T t, int rec, int code, T s, int top) {
   if (!(t > tmin_v<T>())) return;
-  if (t < nb.t) {
+  if (t < nb.t || (t == nb.t && top < nb.top)) {
     if (top != nb.top) {
       nb.t2 = nb.t;
       nb.top2 = nb.top;
@@ -554,16 +560,56 @@ __device__ __forceinline__ int eval_object(const RayT<T>& r, const Rec<T>* recs,
   return i + 1;
 }
 
-// Walks every record for one ray. Inlined, so that the compiler sees which records are in
+// The culling pass for one ray: every object in scene order, or (n_nodes > 0) the objects
+// of the BVH leaves the wave enters. The node index is wave-uniform; a node is entered when
+// some lane's ray meets its box at a t in [tmin, t_best + 4e-5 (1 + t_best)], and an
+// entered leaf's objects are evaluated for every lane, as the full walk evaluates every
+// object. Each object that can hit within that window lies in an entered leaf (boxes are
+// padded beyond the FP32 error), so the winner, and any runner-up that the FP64 refinement
+// looks at (within 1e-5 (1 + t)), are those of the full walk. One loop serves both walks so
+// that eval_object is inlined once. Inlined, so that the compiler sees which records are in
 // LDS and which in global memory.
 template <typename T>
 __device__ __forceinline__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>* recs,
-                                                  const Rec<double>* recs64, const T* verts, int n_recs) {
+                                                  const Rec<double>* recs64, const T* verts, int n_recs,
+                                                  const BvhNode* nodes = nullptr, int n_nodes = 0) {
   Nearest<T> nb;
   nearest_init(nb);
-  int i = 0;
-  while (i < n_recs) i = eval_object(r, recs, recs64, verts, i, nb);
-  return nb;
+  const float ox = (float)r.ox, oy = (float)r.oy, dx = (float)r.dx, dy = (float)r.dy;
+  const float ix = 1.0f / (fabsf(dx) > 1e-30f ? dx : __builtin_copysignf(1e-30f, dx));
+  const float iy = 1.0f / (fabsf(dy) > 1e-30f ? dy : __builtin_copysignf(1e-30f, dy));
+  int rec = 0, node = 0, leaf = 0, k = 0, cnt = 0;
+  for (;;) {
+    int top;
+    if (n_nodes == 0) {
+      if (rec >= n_recs) return nb;
+      top = rec;
+    } else {
+      while (k >= cnt) {
+        if (node >= n_nodes) return nb;
+        const BvhNode& N = nodes[node];
+        const float tx0 = (N.lo[0] - ox) * ix, tx1 = (N.hi[0] - ox) * ix;
+        const float ty0 = (N.lo[1] - oy) * iy, ty1 = (N.hi[1] - oy) * iy;
+        const float tn = fmaxf(fminf(tx0, tx1), fminf(ty0, ty1));
+        const float tf = fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1));
+        const float bt = (float)nb.t;
+        const float lim = bt + 4e-5f * (1.0f + bt);
+        const bool need = tn <= tf && tf >= 1e-5f && tn <= lim;
+        int next = N.skip, c = 0;
+        if (__ballot(need)) {
+          c = N.count;
+          if (!c) next = node + 1;
+        }
+        leaf = __builtin_amdgcn_readfirstlane(node);
+        cnt = __builtin_amdgcn_readfirstlane(c);
+        k = 0;
+        node = __builtin_amdgcn_readfirstlane(next);
+      }
+      top = __builtin_amdgcn_readfirstlane(nodes[leaf].top[k]);
+      k++;
+    }
+    rec = eval_object(r, recs, recs64, verts, top, nb);
+  }
 }
 
 // Outward (closed) or left (open) unit normal of the winning surface at p.
@@ -798,11 +844,67 @@ __device__ __forceinline__ int find_scene(const uint32_t* pre, int n, uint32_t x
   return lo;
 }
 
+// item_pre[s] = first item of scene s for generation `gen_ix`; item_pre[n] = total, run by
+// one workgroup of BLOCK threads. COHERENT: the previous generation's child counts were
+// written by other workgroups of the running kernel, so they are read with device-scope
+// atomic loads (not through this CU's cache).
+template <int BLOCK, bool COHERENT>
+__device__ __forceinline__ void build_items_block(const SceneDesc* __restrict__ descs, int n_scenes,
+                                                  uint32_t gen_ix, uint32_t rpi, uint32_t* item_pre,
+                                                  BatchCtl* ctl) {
+  constexpr int kW = BLOCK / 64;
+  __shared__ uint32_t s_w[kW];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n_scenes; base += BLOCK) {
+    const int sid = base + threadIdx.x;
+    uint32_t n_it = 0;
+    if (sid < n_scenes) {
+      const SceneDesc& D = descs[sid];
+      if (gen_ix < D.max_bounce) {
+        uint64_t c;
+        if (COHERENT && gen_ix > 0) {
+          const uint32_t g = __hip_atomic_load(gen(D.gen_count) + gen_ix - 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+          c = g < D.q_cap ? g : D.q_cap;
+        } else {
+          c = gen_input_count(D, gen_ix);
+        }
+        n_it = (uint32_t)((c + rpi - 1) / rpi);
+      }
+    }
+    // block exclusive scan
+    uint32_t x = n_it;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t wofs = 0;
+    for (int w = 0; w < wave; w++) wofs += s_w[w];
+    const uint32_t carry = s_carry;
+    if (sid < n_scenes) item_pre[sid] = carry + wofs + x - n_it;
+    __syncthreads();
+    if (threadIdx.x == BLOCK - 1) s_carry = carry + wofs + x;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    item_pre[n_scenes] = s_carry;
+    ctl->n_items_gen[gen_ix] = s_carry;
+    ctl->item_cursor[gen_ix] = 0;
+  }
+}
+
 template <typename T, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void trace_kernel(const SceneDesc* __restrict__ descs, int n_scenes,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel(const SceneDesc* __restrict__ descs, int n_scenes,
                                                       uint32_t gen_ix, const uint32_t* __restrict__ item_pre,
                                                       BatchCtl* ctl, uint32_t rpi, int cap_recs,
-                                                      int cap_objs) {
+                                                      int cap_objs, int cap_verts,
+                                                      uint32_t* __restrict__ item_pre_next) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kWaves = BLOCK / 64;
   __shared__ uint32_t s_wseg[kWaves];
@@ -816,6 +918,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const SceneDesc* __restric
   int32_t* s_obj_rec = reinterpret_cast<int32_t*>(s_obj_flags + cap_objs);
   T* s_verts = reinterpret_cast<T*>(
       (reinterpret_cast<uintptr_t>(s_obj_rec + cap_objs) + 15) & ~uintptr_t(15));
+  BvhNode* s_nodes = reinterpret_cast<BvhNode*>(
+      (reinterpret_cast<uintptr_t>(s_verts + 2 * cap_verts) + 15) & ~uintptr_t(15));
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -846,6 +950,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const SceneDesc* __restric
       }
       const T* vsrc = reinterpret_cast<const T*>(gen(D.verts));
       for (int k = threadIdx.x; k < 2 * D.n_verts; k += BLOCK) s_verts[k] = vsrc[k];
+      if (sizeof(T) == 4 && D.n_bvh > 0) {
+        const int4* nsrc = reinterpret_cast<const int4*>(gen(D.bvh));
+        int4* ndst = reinterpret_cast<int4*>(s_nodes);
+        const int n16 = D.n_bvh * (int)(sizeof(BvhNode) / 16);
+        for (int k = threadIdx.x; k < n16; k += BLOCK) ndst[k] = nsrc[k];
+      }
       loaded = sid;
     }
     __syncthreads();
@@ -889,7 +999,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const SceneDesc* __restric
           h = nearest_hit<double>(rd, s_rec, recs64, s_verts, n_recs);
         } else {
           const RayT<float> rf{(float)ox, (float)oy, (float)dx, (float)dy};
-          Nearest<float> nb = nearest_hit<float>(rf, s_rec, recs64, s_verts, n_recs);
+          Nearest<float> nb = nearest_hit<float>(rf, s_rec, recs64, s_verts, n_recs, s_nodes, D.n_bvh);
           nearest_init(h);
           if (nb.top >= 0) refine_fp64(rd, nb, recs64, verts64, n_recs, h);
         }
@@ -1018,60 +1128,40 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const SceneDesc* __restric
       __syncthreads();
     }
   }
+  // The last workgroup to finish builds the next generation's work list (its child counts
+  // are final once every workgroup of this generation is done).
+  if (item_pre_next) {
+    __shared__ bool s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      s_last = atomicAdd(&ctl->gen_done[gen_ix], 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      build_items_block<BLOCK, true>(descs, n_scenes, gen_ix + 1, rpi, item_pre_next, ctl);
+    }
+  }
 }
 
-// item_pre[s] = first item of scene s for generation `gen_ix`; item_pre[n] = total. One block.
 __global__ __launch_bounds__(1024) void build_trace_items_kernel(const SceneDesc* __restrict__ descs,
                                                                  int n_scenes, uint32_t gen_ix, uint32_t rpi,
                                                                  uint32_t* item_pre, BatchCtl* ctl) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (int base = 0; base < n_scenes; base += 1024) {
-    const int sid = base + threadIdx.x;
-    uint32_t n_it = 0;
-    if (sid < n_scenes) {
-      const SceneDesc& D = descs[sid];
-      if (gen_ix < D.max_bounce) {
-        uint64_t c = gen_input_count(D, gen_ix);
-        n_it = (uint32_t)((c + rpi - 1) / rpi);
-      }
-    }
-    // block exclusive scan
-    uint32_t x = n_it;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      uint32_t y = __shfl_up(x, d, 64);
-      if (lane >= d) x += y;
-    }
-    if (lane == 63) s_w[wave] = x;
-    __syncthreads();
-    uint32_t wofs = 0;
-    for (int w = 0; w < wave; w++) wofs += s_w[w];
-    const uint32_t carry = s_carry;
-    if (sid < n_scenes) item_pre[sid] = carry + wofs + x - n_it;
-    __syncthreads();
-    if (threadIdx.x == 1023) s_carry = carry + wofs + x;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    item_pre[n_scenes] = s_carry;
-    ctl->n_items_gen[gen_ix] = s_carry;
-    ctl->item_cursor[gen_ix] = 0;
-  }
+  build_items_block<1024, false>(descs, n_scenes, gen_ix, rpi, item_pre, ctl);
 }
 
 // ---------------------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------------------
 template <typename T>
-size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts) {
+size_t trace_lds_bytes(int n_recs, int n_objs, int n_verts, int n_nodes) {
   size_t b = (size_t)n_recs * sizeof(Rec<T>);
   b += (size_t)n_objs * 8;
   b = (b + 15) & ~size_t(15);
   b += (size_t)n_verts * 2 * sizeof(T);
+  b = (b + 15) & ~size_t(15);
+  b += (size_t)n_nodes * sizeof(BvhNode);
   return b;
 }
 
@@ -1081,11 +1171,11 @@ int trace_block() { return kTraceBlock; }
 
 template <typename T>
 hipError_t launch_trace(const SceneDesc* descs, int n_scenes, uint32_t gen, const uint32_t* item_pre,
-                        BatchCtl* ctl, uint32_t rpi, int cap_recs, int cap_objs, int cap_verts, int grid,
-                        hipStream_t stream) {
-  size_t lds = trace_lds_bytes<T>(cap_recs, cap_objs, cap_verts);
+                        BatchCtl* ctl, uint32_t rpi, int cap_recs, int cap_objs, int cap_verts, int cap_nodes,
+                        uint32_t* item_pre_next, int grid, hipStream_t stream) {
+  size_t lds = trace_lds_bytes<T>(cap_recs, cap_objs, cap_verts, cap_nodes);
   hipLaunchKernelGGL((trace_kernel<T, kTraceBlock>), dim3(grid), dim3(kTraceBlock), lds, stream, descs,
-                     n_scenes, gen, item_pre, ctl, rpi, cap_recs, cap_objs);
+                     n_scenes, gen, item_pre, ctl, rpi, cap_recs, cap_objs, cap_verts, item_pre_next);
   return hipGetLastError();
 }
 
@@ -1097,10 +1187,10 @@ hipError_t launch_build_trace_items(const SceneDesc* descs, int n_scenes, uint32
 }
 
 template hipError_t launch_trace<float>(const SceneDesc*, int, uint32_t, const uint32_t*, BatchCtl*, uint32_t,
-                                        int, int, int, int, hipStream_t);
+                                        int, int, int, int, uint32_t*, int, hipStream_t);
 template hipError_t launch_trace<double>(const SceneDesc*, int, uint32_t, const uint32_t*, BatchCtl*, uint32_t,
-                                         int, int, int, int, hipStream_t);
-template size_t trace_lds_bytes<float>(int, int, int);
-template size_t trace_lds_bytes<double>(int, int, int);
+                                         int, int, int, int, uint32_t*, int, hipStream_t);
+template size_t trace_lds_bytes<float>(int, int, int, int);
+template size_t trace_lds_bytes<double>(int, int, int, int);
 
 }  // namespace lg
