@@ -204,6 +204,8 @@ int lg_ron_write_scene(const lg_object* objects, uint32_t n_objects, const lg_li
 /* Device SoA: pos float4 {x0,y0,x1,y1}; color0 float4; optional color1 float4 (per-vertex
  * colours, string_mod.rs:118-119); key uint64 (ray index << 24 | path bits). */
 enum lg_segment_flags { LG_SEG_VERTEX_COLORS = 1u << 0 };
+/* A buffer holds at most 2^31 - 1 segments: LG_E_OOM for a larger capacity, and for a trace,
+ * upload or string mod that would need more. */
 int lg_segments_create(lg_ctx* ctx, uint64_t capacity, uint32_t flags, lg_segments** out);
 int lg_segments_destroy(lg_segments* segs);
 int lg_segments_count(const lg_segments* segs, uint64_t* count);

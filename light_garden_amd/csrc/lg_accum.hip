@@ -11,10 +11,13 @@
 //      atomics).
 //   2. tiles: a persistent workgroup takes a (tile, pair range) work item, walks each of
 //      its segments' column (x-major) or row (y-major) samples that fall in the tile, one
-//      sample per lane, and adds the two anti-aliased deposits per sample into an FP32
-//      LDS tile (4 colour planes, row stride T+1 so row- and column-walks are free of bank
-//      conflicts). The tile is flushed with coalesced float4 read-add-write when one item
-//      owns the tile, with FP32 atomics when a heavy tile was split across items.
+//      sample per lane, and adds the two anti-aliased deposits per sample into a 64-bit
+//      fixed-point LDS tile (4 colour planes, row stride T+1 so row- and column-walks are
+//      free of bank conflicts). Integer sums make the image independent of the order of the
+//      deposits. When one item owns the tile it converts the sums to FP32 and adds them to
+//      the accumulator with coalesced read-add-write; the items of a heavy tile split over
+//      several items add their exact sums into a 64-bit merge slot, converted afterwards by
+//      the same expression.
 //   3. resolve: FP32 accumulator -> RGBA16F, round to nearest even, alpha + 1 (the
 //      target's clear alpha, renderer.rs:175).
 //
@@ -23,15 +26,17 @@
 // major-axis pixel whose centre k + 0.5 lies in the half-open span [M0, M1); with
 // s = (k + 0.5 - M0) / dM, minor m = m0 + s dm, v = m - 0.5, j = floor(v), f = v - j,
 // pixels j and j + 1 on the minor axis get weights 1 - f and f; a weight w adds w*rgb to
-// RGB and w*a*a to alpha. Per-sample positions are computed in FP64 from the FP32
-// segment end points so that the two tiles sharing a sample agree exactly.
+// RGB and w*a*a to alpha. Each (segment, tile) pair is set up in FP64 from the FP32 end
+// points; its samples are then stepped in FP32 from the tile's first column.
 #include "lg_internal.h"
 
 namespace lg {
 
 constexpr int kTile = 64;
-constexpr int kTileStride = kTile + 1;
-constexpr int kPlane = kTile * kTileStride;
+// One colour plane of the LDS tile, unpadded: element (minor m, major q) of a walk sits at
+// m * kTile + q, so the 64 lanes of a deposit (consecutive q) fall in distinct banks whatever
+// the segment's slope.
+constexpr int kPlane = kTile * kTile;
 constexpr int kTileBlock = 1024;
 
 struct SegGeom {
@@ -40,8 +45,10 @@ struct SegGeom {
   int xmajor;
 };
 
-// Returns false when the segment deposits nothing.
-__device__ __forceinline__ bool seg_geom(const float4 p, const AccumView& v, SegGeom& g) {
+// Returns false when the segment deposits nothing, or when `want_xmajor` is 0 or 1 and the
+// segment's major axis is the other one.
+__device__ __forceinline__ bool seg_geom(const float4 p, const AccumView& v, SegGeom& g,
+                                         int want_xmajor = -1) {
   double X0 = ((double)p.x - v.left) * v.sx;
   double Y0 = (v.top - (double)p.y) * v.sy;
   double X1 = ((double)p.z - v.left) * v.sx;
@@ -49,6 +56,7 @@ __device__ __forceinline__ bool seg_geom(const float4 p, const AccumView& v, Seg
   if (!isfinite(X0) || !isfinite(Y0) || !isfinite(X1) || !isfinite(Y1)) return false;
   double dX = X1 - X0, dY = Y1 - Y0;
   int xm = fabs(dX) >= fabs(dY);
+  if (want_xmajor >= 0 && xm != want_xmajor) return false;
   double M0 = xm ? X0 : Y0, M1 = xm ? X1 : Y1;
   double m0 = xm ? Y0 : X0, dM = xm ? dX : dY, dm = xm ? dY : dX;
   double lo, hi;
@@ -270,7 +278,7 @@ __global__ __launch_bounds__(256) void bin_kernel(const SceneDesc* __restrict__ 
       } else {
         for_each_tile(g, v, [&](int tile) {
           uint32_t slot = atomicAdd(&s_hist[tile], 1u);
-          if (slot < pairs_cap) pairs[slot] = (uint32_t)s;
+          if (slot < pairs_cap) pairs[slot] = (uint32_t)s | ((uint32_t)g.xmajor << 31);
         });
       }
     }
@@ -444,22 +452,28 @@ __global__ __launch_bounds__(256) void sum_u64_kernel(const uint32_t* in, uint64
 constexpr int kWBits = 20;
 constexpr int kWOne = 1 << kWBits;
 constexpr int kSlotU64 = 4 * kTile * kTile;  // one merge slot: 4 channel planes of a tile
+// A pair entry is the segment's index in its scene with the segment's major axis in bit 31
+// (1 = x); kNoPair marks lanes past the end of an item.
+constexpr uint32_t kPairIndex = 0x7FFFFFFFu;
+constexpr uint32_t kNoPair = 0xFFFFFFFFu;
 
-// G of a scene's fixed point, from the scene's colour bound and its segment count only (never
-// from a buffer capacity, so it is the same alone or in any batch). A segment deposits at most
-// once into a given pixel, so every partial sum of a pixel is bounded by
-// n * 2^20 * max|cq|; with n <= 2^lb and G + log2(bound) <= 41 - lb that stays below 2^62,
-// and cq below 2^30. bound = max|colour| * max|blend factor| (max(m, m^2) for the default
-// One / SrcAlpha blend).
-__device__ __forceinline__ int scene_fixed_bits(const SceneDesc& D) {
+// G of channel c of a scene's fixed point, from the scene's colour bound for that channel and
+// its segment count only (never from a buffer capacity, so it is the same alone or in any
+// batch). Each channel has its own scale: with the default blend alpha deposits a * a while
+// RGB deposit the colour, so for bright colours the two differ by many orders of magnitude.
+// A segment deposits at most once into a given pixel, so every partial sum of a pixel is
+// bounded by n * 2^20 * max|cq|; with n <= 2^lb and G + log2(bound) <= 41 - lb that stays
+// below 2^62, and |cq| <= 2^29. bound = max|colour| * max|blend factor of the channel|.
+__device__ __forceinline__ int scene_fixed_bits(const SceneDesc& D, int c) {
   const double m = (double)D.color_max;
-  const double bound = m * (double)D.factor_max;
+  const double bound = m * (double)D.factor_max[c];
   if (!(bound > 0.0) || !(bound < 1e300)) return 30;
   const uint64_t n = scene_segments(D);
   const int lb = n > 1 ? 64 - __clzll((long long)(n - 1)) : 0;  // n <= 2^lb
   const double room = (42 - lb) < 30 ? (double)(42 - lb) : 30.0;
   const int G = (int)floor(room - log2(bound)) - 1;
-  return G < -30 ? -30 : (G > 60 ? 60 : G);
+  // 2^G stays a finite float; below 2^-149 it is 0 and such colours deposit nothing
+  return G < -250 ? -250 : (G > 100 ? 100 : G);
 }
 
 // The value one sample of colour channel c adds per unit weight: +-c * factor (BlendCode).
@@ -479,15 +493,6 @@ __device__ __forceinline__ float blend_value(uint32_t code, float c, float a, fl
   return (code & BC_NEGATE) ? -v : v;
 }
 
-__device__ __forceinline__ void lds_deposit(unsigned long long* tile, int row, int col, int w,
-                                            const int cq[4]) {
-  const int idx = row * kTileStride + col;
-  atomicAdd(&tile[idx], (unsigned long long)((long long)w * (long long)cq[0]));
-  atomicAdd(&tile[kPlane + idx], (unsigned long long)((long long)w * (long long)cq[1]));
-  atomicAdd(&tile[2 * kPlane + idx], (unsigned long long)((long long)w * (long long)cq[2]));
-  atomicAdd(&tile[3 * kPlane + idx], (unsigned long long)((long long)w * (long long)cq[3]));
-}
-
 __device__ __forceinline__ int bcast_i(int v, int lane) {
   return __builtin_amdgcn_readlane(v, lane);
 }
@@ -503,10 +508,39 @@ __device__ __forceinline__ double bcast_d(double v, int lane) {
 
 __device__ __forceinline__ int fx(float v, float scale) { return (int)__builtin_rintf(v * scale); }
 
+// In-place transpose of the four 64x64 planes, (r, c) <-> (c, r). Job j of 128 is plane j / 32
+// and diagonal pair d = j % 32 + 1: lanes 0 .. 63 - d swap the elements of diagonal c - r = d,
+// lanes 64 - d .. 63 those of diagonal 64 - d (d < 32; diagonal 32 takes lanes 0 .. 31 alone).
+// Every element off the main diagonal is swapped exactly once, and the lanes' addresses step
+// by 65 elements, which keeps the swaps free of bank conflicts.
+template <int WAVES>
+__device__ __forceinline__ void transpose_tile(unsigned long long* t, int wave, int lane) {
+  for (int j = wave; j < 4 * 32; j += WAVES) {
+    const int d = (j & 31) + 1;
+    unsigned long long* pl = t + (j >> 5) * kPlane;
+    const bool first_diag = lane < kTile - d;
+    const int r = first_diag ? lane : lane - (kTile - d);
+    const int c = first_diag ? lane + d : r + (kTile - d);
+    if (d < 32 || lane < 32) {
+      const int a = r * kTile + c, b = c * kTile + r;
+      const unsigned long long va = pl[a], vb = pl[b];
+      pl[a] = vb;
+      pl[b] = va;
+    }
+  }
+}
+
 // One persistent workgroup per (tile, pair range) item. Each wave takes 64 pairs at a time:
 // lane l loads pair l's segment and computes its sample range inside the tile plus the
 // tile-relative FP32 parameters of its samples; the wave then walks those pairs one after
 // another with one sample per lane, reading the pair's setup from scalar registers.
+//
+// An item is walked twice: first its y-major pairs with the tile held transposed (pixel
+// (row y, column x) at x * 64 + y), then, after an in-place transpose, its x-major pairs with
+// the tile row-major (y * 64 + x). Either way a deposit of the sample at major offset q and
+// minor pixel m lands at m * 64 + q: the lanes of one deposit instruction cover consecutive
+// addresses for every slope (a single padded layout serves one orientation's walks without
+// conflicts but stacks the lanes of the other's diagonal walks in a few banks).
 //
 // Sample positions are anchored on the tile column (the first major-axis pixel of the
 // tile), which is the same for every tile that shares a sample: m(q) = m_anchor + q * slope,
@@ -520,6 +554,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
                                                           unsigned long long* split_fx, BatchCtl* ctl) {
   __shared__ unsigned long long s_tile[4 * kPlane];
   __shared__ uint32_t s_item;
+  __shared__ uint32_t s_batch[2];  // next batch of 64 pairs, per phase
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   constexpr int kWaves = BLOCK / 64;
@@ -541,183 +576,226 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
     const float4* __restrict__ col0 = gen(D.seg_col);
     const float4* __restrict__ col1 = gen(D.seg_col1);
     float4* accum = gen(D.accum);
-    const int G = scene_fixed_bits(D);
-    const float cscale = __builtin_ldexpf(1.0f, G);
+    int G[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) G[c] = scene_fixed_bits(D, c);
+    const float cs0 = __builtin_ldexpf(1.0f, G[0]), cs1 = __builtin_ldexpf(1.0f, G[1]);
+    const float cs2 = __builtin_ldexpf(1.0f, G[2]), cs3 = __builtin_ldexpf(1.0f, G[3]);
     const uint32_t bc = __builtin_amdgcn_readfirstlane(D.blend_code);
     const uint32_t bc0 = bc & 15u, bc1 = (bc >> 4) & 15u, bc2 = (bc >> 8) & 15u, bc3 = (bc >> 12) & 15u;
     const float bk0 = D.blend_k[0], bk1 = D.blend_k[1], bk2 = D.blend_k[2], bk3 = D.blend_k[3];
-    const double unscale = __builtin_ldexp(1.0, -(G + kWBits));
     for (int k = threadIdx.x; k < 4 * kPlane; k += BLOCK) s_tile[k] = 0ull;
+    if (threadIdx.x == 0) s_batch[0] = s_batch[1] = 0u;
     __syncthreads();
     const int tx = (int)(W.tile % (uint32_t)v.tiles_x);
     const int ty = (int)(W.tile / (uint32_t)v.tiles_x);
     const int x0 = tx * kTile, y0 = ty * kTile;
-    for (uint32_t base = W.begin + wave * 64; base < W.end; base += kWaves * 64) {
-      // ---- per-lane setup of one pair ----
-      const uint32_t pi = base + lane;
-      int packed = 0;  // n | q0 << 8 | xmajor << 16 | vertex colours << 17 | negative colour << 18
-      float m_anchor = 0.f, slope = 0.f, s_anchor = 0.f, ds = 0.f;
-      float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), dc = ca;
-      int cq0 = 0, cq1 = 0, cq2 = 0, cq3 = 0;
-      if (pi < W.end) {
-        const uint32_t s = pairs[pi];
-        SegGeom g;
-        if (seg_geom(pos[s], v, g)) {
-          const long maj0 = g.xmajor ? x0 : y0, min0 = g.xmajor ? y0 : x0;
-          long a = g.klo > maj0 ? g.klo : maj0;
-          long b = g.khi < maj0 + kTile - 1 ? g.khi : maj0 + kTile - 1;
-          if (a <= b) minor_window(g, min0, min0 + kTile - 1, a, b);
-          if (a <= b) {
-            const double u0 = ((double)maj0 + 0.5) - g.M0;
-            // minor coordinate of the column's sample, tile-relative, minus the pixel-centre 0.5
-            m_anchor = (float)(g.m0 + u0 * g.slope - (double)min0 - 0.5);
-            slope = (float)g.slope;
-            ca = col0[s];
-            const float4 cb = col1 ? col1[s] : ca;
-            dc = make_float4(cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w);
-            const int vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
-            s_anchor = (float)(u0 * g.inv_dM);
-            ds = (float)g.inv_dM;
-            cq0 = fx(blend_value(bc0, ca.x, ca.w, bk0), cscale);
-            cq1 = fx(blend_value(bc1, ca.y, ca.w, bk1), cscale);
-            cq2 = fx(blend_value(bc2, ca.z, ca.w, bk2), cscale);
-            cq3 = fx(blend_value(bc3, ca.w, ca.w, bk3), cscale);
-            const int neg = (cq0 | cq1 | cq2 | cq3) < 0 ? 1 : 0;
-            packed = (int)(b - a + 1) | ((int)(a - maj0) << 8) | (g.xmajor << 16) | (vc << 17) | (neg << 18);
+    const uint32_t n_batches = (W.end - W.begin + 63u) >> 6;
+    // pair entry of this lane in batch b, kNoPair past the item's end
+    auto entry = [&](uint32_t b) -> uint32_t {
+      const uint32_t pi = W.begin + b * 64u + (uint32_t)lane;
+      return b < n_batches && pi < W.end ? pairs[pi] : kNoPair;
+    };
+    for (int phase = 0; phase < 2; phase++) {  // 0: y-major pairs, 1: x-major pairs
+      // Batches of 64 pairs come from the phase's counter in LDS, so the waves of an item end
+      // a phase together whatever the lengths of their pairs' runs. Each wave works two
+      // batches ahead: while it walks batch b0, the end points and colours of b1 and the
+      // pair entries of b2 are in flight.
+      auto fetch = [&](uint32_t e, float4& p, float4& ca, float4& cb) {
+        if (e != kNoPair && (e >> 31) == (uint32_t)phase) {
+          const uint32_t s = e & kPairIndex;
+          p = pos[s];
+          ca = col0[s];
+          cb = col1 ? col1[s] : ca;
+        }
+      };
+      uint32_t taken = 0;  // lane 0: the counter's previous value
+      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
+      uint32_t b0 = __builtin_amdgcn_readfirstlane(taken);
+      uint32_t e0 = entry(b0);
+      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
+      uint32_t b1 = __builtin_amdgcn_readfirstlane(taken);
+      uint32_t e1 = entry(b1);
+      float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), a0 = p0, c0v = p0;
+      fetch(e0, p0, a0, c0v);
+      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
+      while (b0 < n_batches) {
+        // ---- per-lane setup of one pair of batch b0 ----
+        int packed = 0;  // n | q0 << 8 | xmajor << 16 | vertex colours << 17 | negative colour << 18
+        float m_anchor = 0.f, slope = 0.f, s_anchor = 0.f, ds = 0.f;
+        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), dc = ca;
+        int cq0 = 0, cq1 = 0, cq2 = 0, cq3 = 0;
+        if (e0 != kNoPair && (e0 >> 31) == (uint32_t)phase) {
+          SegGeom g;
+          if (seg_geom(p0, v, g, phase)) {
+            const long maj0 = g.xmajor ? x0 : y0, min0 = g.xmajor ? y0 : x0;
+            long a = g.klo > maj0 ? g.klo : maj0;
+            long b = g.khi < maj0 + kTile - 1 ? g.khi : maj0 + kTile - 1;
+            if (a <= b) minor_window(g, min0, min0 + kTile - 1, a, b);
+            if (a <= b) {
+              const double u0 = ((double)maj0 + 0.5) - g.M0;
+              // minor coordinate of the column's sample, tile-relative, minus the pixel-centre 0.5
+              m_anchor = (float)(g.m0 + u0 * g.slope - (double)min0 - 0.5);
+              slope = (float)g.slope;
+              ca = a0;
+              const float4 cb = c0v;
+              dc = make_float4(cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w);
+              const int vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
+              s_anchor = (float)(u0 * g.inv_dM);
+              ds = (float)g.inv_dM;
+              cq0 = fx(blend_value(bc0, ca.x, ca.w, bk0), cs0);
+              cq1 = fx(blend_value(bc1, ca.y, ca.w, bk1), cs1);
+              cq2 = fx(blend_value(bc2, ca.z, ca.w, bk2), cs2);
+              cq3 = fx(blend_value(bc3, ca.w, ca.w, bk3), cs3);
+              const int neg = (cq0 | cq1 | cq2 | cq3) < 0 ? 1 : 0;
+              packed = (int)(b - a + 1) | ((int)(a - maj0) << 8) | (g.xmajor << 16) | (vc << 17) | (neg << 18);
+            }
           }
         }
+        // ---- loads for the batches ahead ----
+        const uint32_t b2 = __builtin_amdgcn_readfirstlane(taken);
+        float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = p1, c1v = p1;
+        fetch(e1, p1, a1, c1v);
+        const uint32_t e2 = entry(b2);
+        if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
+        // ---- walk the pairs that have samples ----
+        // Pair j's samples are columns jq0 .. jq0 + jn - 1 of the tile (jn <= 64): lane t takes
+        // column jq0 + t. Deposits are exact 64-bit products: (2^20 - wq) * c at minor jl and
+        // wq * c at jl + 1, which add up to exactly c << 20.
+        uint64_t todo = __ballot(packed != 0);
+        while (todo) {
+          const int j = __builtin_ctzll(todo);
+          todo &= todo - 1;
+          const int jp = bcast_i(packed, j);
+          const int jn = jp & 0xff, jq0 = (jp >> 8) & 0xff, jvc = (jp >> 17) & 1;
+          const int jneg = (jp >> 18) & 1;
+          wave_samples += (unsigned long long)jn;
+          const float jm = bcast_f(m_anchor, j), jsl = bcast_f(slope, j);
+          const int q = jq0 + lane;
+          if (!jvc && !jneg) {
+            // non-negative colour: unsigned 32x32 -> 64-bit products (one multiply per deposit)
+            const unsigned jc0 = (unsigned)bcast_i(cq0, j), jc1 = (unsigned)bcast_i(cq1, j),
+                           jc2 = (unsigned)bcast_i(cq2, j), jc3 = (unsigned)bcast_i(cq3, j);
+            if (lane < jn) {
+              const float m = __builtin_fmaf((float)q, jsl, jm);
+              const float jf = __builtin_floorf(m);
+              const int jl = (int)jf;
+              int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
+              wq = wq > kWOne ? kWOne : wq;
+              const unsigned uq = (unsigned)wq, ul = (unsigned)(kWOne - wq);
+              const int i1 = (jl + 1) * kTile + q;  // minor jl + 1, major q
+              const int i0 = i1 - kTile;
+              if ((unsigned)jl < (unsigned)kTile) {
+                atomicAdd(&s_tile[i0], (unsigned long long)ul * jc0);
+                atomicAdd(&s_tile[kPlane + i0], (unsigned long long)ul * jc1);
+                atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)ul * jc2);
+                atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)ul * jc3);
+              }
+              if ((unsigned)(jl + 1) < (unsigned)kTile) {
+                atomicAdd(&s_tile[i1], (unsigned long long)uq * jc0);
+                atomicAdd(&s_tile[kPlane + i1], (unsigned long long)uq * jc1);
+                atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)uq * jc2);
+                atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)uq * jc3);
+              }
+            }
+          } else if (!jvc) {
+            const int jc0 = bcast_i(cq0, j), jc1 = bcast_i(cq1, j), jc2 = bcast_i(cq2, j), jc3 = bcast_i(cq3, j);
+            if (lane < jn) {
+              const float m = __builtin_fmaf((float)q, jsl, jm);
+              const float jf = __builtin_floorf(m);
+              const int jl = (int)jf;  // tile-relative minor index of the first deposit pixel
+              int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
+              wq = wq > kWOne ? kWOne : wq;
+              const int wl = kWOne - wq;
+              const int i1 = (jl + 1) * kTile + q;  // minor jl + 1, major q
+              const int i0 = i1 - kTile;
+              if ((unsigned)jl < (unsigned)kTile) {
+                atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * jc0));
+                atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * jc1));
+                atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * jc2));
+                atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * jc3));
+              }
+              if ((unsigned)(jl + 1) < (unsigned)kTile) {
+                atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * jc0));
+                atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * jc1));
+                atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * jc2));
+                atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * jc3));
+              }
+            }
+          } else {
+            // per-vertex colours: the colour is interpolated along the segment at each sample
+            const float jds = bcast_f(ds, j), jsa = bcast_f(s_anchor, j);
+            const float a0 = bcast_f(ca.x, j), a1 = bcast_f(ca.y, j), a2 = bcast_f(ca.z, j), a3 = bcast_f(ca.w, j);
+            const float d0 = bcast_f(dc.x, j), d1 = bcast_f(dc.y, j), d2 = bcast_f(dc.z, j), d3 = bcast_f(dc.w, j);
+            if (lane < jn) {
+              const float m = __builtin_fmaf((float)q, jsl, jm);
+              const float jf = __builtin_floorf(m);
+              const int jl = (int)jf;
+              int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
+              wq = wq > kWOne ? kWOne : wq;
+              const int wl = kWOne - wq;
+              const float sp = __builtin_fmaf((float)q, jds, jsa);
+              const float al = __builtin_fmaf(sp, d3, a3);
+              const int c0 = fx(blend_value(bc0, __builtin_fmaf(sp, d0, a0), al, bk0), cs0);
+              const int c1 = fx(blend_value(bc1, __builtin_fmaf(sp, d1, a1), al, bk1), cs1);
+              const int c2 = fx(blend_value(bc2, __builtin_fmaf(sp, d2, a2), al, bk2), cs2);
+              const int c3 = fx(blend_value(bc3, al, al, bk3), cs3);
+              const int i1 = (jl + 1) * kTile + q;  // minor jl + 1, major q
+              const int i0 = i1 - kTile;
+              if ((unsigned)jl < (unsigned)kTile) {
+                atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * c0));
+                atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * c1));
+                atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * c2));
+                atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * c3));
+              }
+              if ((unsigned)(jl + 1) < (unsigned)kTile) {
+                atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * c0));
+                atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * c1));
+                atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * c2));
+                atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * c3));
+              }
+            }
+          }
+        }
+        b0 = b1;
+        e0 = e1;
+        p0 = p1;
+        a0 = a1;
+        c0v = c1v;
+        b1 = b2;
+        e1 = e2;
       }
-      // ---- walk the pairs that have samples ----
-      // Pair j's samples are columns jq0 .. jq0 + jn - 1 of the tile (jn <= 64): lane t takes
-      // column jq0 + t. Deposits are 32x32 -> 64-bit products: (2^20 - wq) * c at minor jl and
-      // wq * c at jl + 1, which add up to exactly c << 20.
-      uint64_t todo = __ballot(packed != 0);
-      while (todo) {
-        const int j = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        const int jp = bcast_i(packed, j);
-        const int jn = jp & 0xff, jq0 = (jp >> 8) & 0xff, jxm = (jp >> 16) & 1, jvc = (jp >> 17) & 1;
-        const int jneg = (jp >> 18) & 1;
-        wave_samples += (unsigned long long)jn;
-        const float jm = bcast_f(m_anchor, j), jsl = bcast_f(slope, j);
-        // tile index of (major q, minor r) = r * rstep + q * qstep
-        const int rstep = jxm ? kTileStride : 1;
-        const int qstep = jxm ? 1 : kTileStride;
-        const int q = jq0 + lane;
-        if (!jvc && !jneg) {
-          // non-negative colour: unsigned 32x32 -> 64-bit products (one multiply per deposit)
-          const unsigned jc0 = (unsigned)bcast_i(cq0, j), jc1 = (unsigned)bcast_i(cq1, j),
-                         jc2 = (unsigned)bcast_i(cq2, j), jc3 = (unsigned)bcast_i(cq3, j);
-          if (lane < jn) {
-            const float m = __builtin_fmaf((float)q, jsl, jm);
-            const float jf = __builtin_floorf(m);
-            const int jl = (int)jf;
-            int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
-            wq = wq > kWOne ? kWOne : wq;
-            const unsigned uq = (unsigned)wq, ul = (unsigned)(kWOne - wq);
-            // 24-bit multiplies (jl >= -1 wherever a deposit is made); i1 = i0 + rstep
-            const int i1 = (int)(__umul24((unsigned)(jl + 1), (unsigned)rstep) + __umul24((unsigned)q, (unsigned)qstep));
-            const int i0 = i1 - rstep;
-            if ((unsigned)jl < (unsigned)kTile) {
-              atomicAdd(&s_tile[i0], (unsigned long long)ul * jc0);
-              atomicAdd(&s_tile[kPlane + i0], (unsigned long long)ul * jc1);
-              atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)ul * jc2);
-              atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)ul * jc3);
-            }
-            if ((unsigned)(jl + 1) < (unsigned)kTile) {
-              atomicAdd(&s_tile[i1], (unsigned long long)uq * jc0);
-              atomicAdd(&s_tile[kPlane + i1], (unsigned long long)uq * jc1);
-              atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)uq * jc2);
-              atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)uq * jc3);
-            }
-          }
-        } else if (!jvc) {
-          const int jc0 = bcast_i(cq0, j), jc1 = bcast_i(cq1, j), jc2 = bcast_i(cq2, j), jc3 = bcast_i(cq3, j);
-          if (lane < jn) {
-            const float m = __builtin_fmaf((float)q, jsl, jm);
-            const float jf = __builtin_floorf(m);
-            const int jl = (int)jf;  // tile-relative minor index of the first deposit pixel
-            int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
-            wq = wq > kWOne ? kWOne : wq;
-            const int wl = kWOne - wq;
-            // 24-bit multiplies (jl >= -1 wherever a deposit is made); i1 = i0 + rstep
-            const int i1 = (int)(__umul24((unsigned)(jl + 1), (unsigned)rstep) + __umul24((unsigned)q, (unsigned)qstep));
-            const int i0 = i1 - rstep;
-            if ((unsigned)jl < (unsigned)kTile) {
-              atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * jc0));
-              atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * jc1));
-              atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * jc2));
-              atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * jc3));
-            }
-            if ((unsigned)(jl + 1) < (unsigned)kTile) {
-              atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * jc0));
-              atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * jc1));
-              atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * jc2));
-              atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * jc3));
-            }
-          }
-        } else {
-          // per-vertex colours: the colour is interpolated along the segment at each sample
-          const float jds = bcast_f(ds, j), jsa = bcast_f(s_anchor, j);
-          const float a0 = bcast_f(ca.x, j), a1 = bcast_f(ca.y, j), a2 = bcast_f(ca.z, j), a3 = bcast_f(ca.w, j);
-          const float d0 = bcast_f(dc.x, j), d1 = bcast_f(dc.y, j), d2 = bcast_f(dc.z, j), d3 = bcast_f(dc.w, j);
-          if (lane < jn) {
-            const float m = __builtin_fmaf((float)q, jsl, jm);
-            const float jf = __builtin_floorf(m);
-            const int jl = (int)jf;
-            int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
-            wq = wq > kWOne ? kWOne : wq;
-            const int wl = kWOne - wq;
-            const float sp = __builtin_fmaf((float)q, jds, jsa);
-            const float al = __builtin_fmaf(sp, d3, a3);
-            const int c0 = fx(blend_value(bc0, __builtin_fmaf(sp, d0, a0), al, bk0), cscale);
-            const int c1 = fx(blend_value(bc1, __builtin_fmaf(sp, d1, a1), al, bk1), cscale);
-            const int c2 = fx(blend_value(bc2, __builtin_fmaf(sp, d2, a2), al, bk2), cscale);
-            const int c3 = fx(blend_value(bc3, al, al, bk3), cscale);
-            // 24-bit multiplies (jl >= -1 wherever a deposit is made); i1 = i0 + rstep
-            const int i1 = (int)(__umul24((unsigned)(jl + 1), (unsigned)rstep) + __umul24((unsigned)q, (unsigned)qstep));
-            const int i0 = i1 - rstep;
-            if ((unsigned)jl < (unsigned)kTile) {
-              atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * c0));
-              atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * c1));
-              atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * c2));
-              atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * c3));
-            }
-            if ((unsigned)(jl + 1) < (unsigned)kTile) {
-              atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * c0));
-              atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * c1));
-              atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * c2));
-              atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * c3));
-            }
-          }
-        }
+      __syncthreads();
+      if (phase == 0) {
+        transpose_tile<kWaves>(s_tile, wave, lane);
+        __syncthreads();
       }
     }
-    __syncthreads();
     if (split) {
       // exact partial sums into the tile's merge slot (integer atomics: order independent)
       unsigned long long* fx = split_fx + (size_t)tile_split[D.tile_base + W.tile] * kSlotU64;
       for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
-        const int idx = (k / kTile) * kTileStride + (k % kTile);
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-          const unsigned long long val = s_tile[c * kPlane + idx];
+          const unsigned long long val = s_tile[c * kPlane + k];
           if (val) atomicAdd(&fx[c * kTile * kTile + k], val);
         }
       }
     } else {
       // the only item of its tile: fixed point -> FP32 accumulator
+      const double un0 = __builtin_ldexp(1.0, -(G[0] + kWBits)), un1 = __builtin_ldexp(1.0, -(G[1] + kWBits));
+      const double un2 = __builtin_ldexp(1.0, -(G[2] + kWBits)), un3 = __builtin_ldexp(1.0, -(G[3] + kWBits));
       for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
         int r = k / kTile, cc = k % kTile;
         int gx = x0 + cc, gy = y0 + r;
         if (gx >= v.width || gy >= v.height) continue;
-        int idx = r * kTileStride + cc;
+        const int idx = k;  // row-major
         float4* dst = accum + (size_t)gy * v.width + gx;
         float4 o = *dst;
-        o.x += (float)((double)(long long)s_tile[idx] * unscale);
-        o.y += (float)((double)(long long)s_tile[kPlane + idx] * unscale);
-        o.z += (float)((double)(long long)s_tile[2 * kPlane + idx] * unscale);
-        o.w += (float)((double)(long long)s_tile[3 * kPlane + idx] * unscale);
+        o.x += (float)((double)(long long)s_tile[idx] * un0);
+        o.y += (float)((double)(long long)s_tile[kPlane + idx] * un1);
+        o.z += (float)((double)(long long)s_tile[2 * kPlane + idx] * un2);
+        o.w += (float)((double)(long long)s_tile[3 * kPlane + idx] * un3);
         *dst = o;
       }
     }
@@ -742,7 +820,9 @@ __global__ __launch_bounds__(256) void tile_merge_kernel(const SceneDesc* __rest
     const AccumView v = D.view;
     const uint32_t t = g - (uint32_t)D.tile_base;
     const int x0 = (int)(t % (uint32_t)v.tiles_x) * kTile, y0 = (int)(t / (uint32_t)v.tiles_x) * kTile;
-    const double unscale = __builtin_ldexp(1.0, -(scene_fixed_bits(D) + kWBits));
+    double un[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) un[c] = __builtin_ldexp(1.0, -(scene_fixed_bits(D, c) + kWBits));
     unsigned long long* fx = split_fx + (size_t)sl * kSlotU64;
     for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
       unsigned long long a[4];
@@ -756,10 +836,10 @@ __global__ __launch_bounds__(256) void tile_merge_kernel(const SceneDesc* __rest
       if (gx >= v.width || gy >= v.height) continue;
       float4* dst = gen(D.accum) + (size_t)gy * v.width + gx;
       float4 o = *dst;
-      o.x += (float)((double)(long long)a[0] * unscale);
-      o.y += (float)((double)(long long)a[1] * unscale);
-      o.z += (float)((double)(long long)a[2] * unscale);
-      o.w += (float)((double)(long long)a[3] * unscale);
+      o.x += (float)((double)(long long)a[0] * un[0]);
+      o.y += (float)((double)(long long)a[1] * un[1]);
+      o.z += (float)((double)(long long)a[2] * un[2]);
+      o.w += (float)((double)(long long)a[3] * un[3]);
       *dst = o;
     }
   }

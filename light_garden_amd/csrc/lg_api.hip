@@ -533,6 +533,7 @@ int compile_scene(lg_scene* sc, hipStream_t st) {
 
 int seg_ensure(lg_segments* s, uint64_t cap, bool need64) {
   int rc;
+  if (cap > kMaxSegments) return fail(LG_E_OOM, "segments: at most 2^31 - 1 in one buffer");
   if (cap > s->cap || !s->pos.p) {
     if ((rc = s->pos.ensure(cap * sizeof(float4)))) return rc;
     if ((rc = s->col0.ensure(cap * sizeof(float4)))) return rc;

@@ -11,8 +11,7 @@ using lg::fail;
 
 namespace lg {
 
-float blend_factor_bound(uint32_t code, float m, const float k[4]) {
-  float best = 0.0f;
+void blend_factor_bounds(uint32_t code, float m, const float k[4], float out[4]) {
   for (int c = 0; c < 4; c++) {
     float f;
     switch ((code >> (4 * c)) & 7u) {
@@ -25,9 +24,8 @@ float blend_factor_bound(uint32_t code, float m, const float k[4]) {
       case BC_CONST: f = std::fabs(k[c]); break;
       default: f = std::fabs(1.0f - k[c]); break;
     }
-    best = f > best ? f : best;
+    out[c] = f;
   }
-  return best;
 }
 
 }  // namespace lg

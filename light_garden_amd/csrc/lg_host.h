@@ -195,8 +195,8 @@ int check_params(const lg_params* p);
 void ctx_child_gone(lg_ctx* ctx);
 void begin_call(lg_ctx* ctx);
 void end_call(lg_ctx* ctx);
-// max |blend factor| of a BlendCode word over colours with |channel| <= m
-float blend_factor_bound(uint32_t code, float m, const float k[4]);
+// per channel, max |blend factor| of a BlendCode word over colours with |channel| <= m
+void blend_factor_bounds(uint32_t code, float m, const float k[4], float out[4]);
 
 // Runs `stages` for every entry as one batched frame on ctx's stream and waits for it.
 int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& entries, uint32_t stages);

@@ -120,6 +120,8 @@ constexpr int kMaxRecords = 1024;      // records resident in LDS per trace work
 constexpr int kMaxObjects = 1024;
 constexpr int kMaxPolyVertices = 1024; // vertices of all polygons in a scene
 constexpr int kMaxLights = 64;
+// Segments in one buffer: the accumulation's pair entries hold a segment index in 31 bits.
+constexpr uint64_t kMaxSegments = (1ull << 31) - 1;
 
 // ---- lights --------------------------------------------------------------------------
 struct DevLight {
@@ -217,10 +219,10 @@ struct SceneDesc {
   uint64_t px_base;           // first pixel of this scene in the batch's pixel range
   // blend of the image (lg_image_set_blend): per channel (r, g, b, a) 4 bits at 4 * channel,
   // a BlendCode factor in the low 3 bits and bit 3 set for ReverseSubtract; the constant;
-  // the largest |factor| over |colour| <= color_max
+  // per channel, the largest |factor| over |colour| <= color_max
   uint32_t blend_code;
   float blend_k[4];
-  float factor_max;
+  float factor_max[4];
 };
 
 // Source factors of the supported blends, per channel c of the interpolated colour (a = its

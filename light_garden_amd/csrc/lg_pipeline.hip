@@ -396,7 +396,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
     D.color_max = max_abs;
     D.blend_code = e.img ? e.img->blend_code : kBlendDefault;
     for (int c = 0; c < 4; c++) D.blend_k[c] = e.img ? e.img->blend.constant[c] : 0.0f;
-    D.factor_max = blend_factor_bound(D.blend_code, max_abs, D.blend_k);
+    blend_factor_bounds(D.blend_code, max_abs, D.blend_k, D.factor_max);
     D.tile_base = L.tile_base[i];
     if (accum) {
       int t = D.view.tiles_x * D.view.tiles_y;

@@ -69,6 +69,23 @@ def test_vertex_colour_interpolation(ctx, orc):
     assert orc.rel_l1(img.accum(), ref) < 1e-5
 
 
+@pytest.mark.parametrize("scale", [1e-15, 1.0, 1e15])
+def test_colour_scales(ctx, orc, scale):
+    """The fixed point follows the colour bound of the segments: colours fifteen orders of
+    magnitude from 1 (alpha deposits a^2, up to 1e30) keep full precision, RGB against the
+    FP64 oracle."""
+    rng = np.random.default_rng(9)
+    n = 2000
+    pos = rng.uniform(-1.2, 1.2, (n, 4)).astype(np.float32).astype(np.float64)
+    col = (rng.uniform(0, 1, (n, 4)) * scale).astype(np.float32)
+    img = gpu_accum(ctx, pos, col, None, 256, 256)
+    view = abi.LgView(-1.0, 1.0, -1.0, 1.0, 256, 256)
+    ref = orc.accumulate_f64(pos, col, None, view)
+    acc = img.accum()
+    assert np.isfinite(acc).all()
+    assert orc.rel_l1(acc, ref) < 1e-5
+
+
 def test_edge_cases(ctx, orc):
     """Zero-length, axis-aligned, pixel-centre-aligned, off-screen, partly off-screen, exact
     diagonals, NaN end points and segments spanning many tiles."""
