@@ -606,28 +606,23 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
           const uint32_t s = e & kPairIndex;
           p = pos[s];
           ca = col0[s];
-          cb = col1 ? col1[s] : ca;
+          if (col1) cb = col1[s];
         }
       };
       uint32_t taken = 0;  // lane 0: the counter's previous value
-      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
-      uint32_t b0 = __builtin_amdgcn_readfirstlane(taken);
-      uint32_t e0 = entry(b0);
-      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
-      uint32_t b1 = __builtin_amdgcn_readfirstlane(taken);
-      uint32_t e1 = entry(b1);
-      float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), a0 = p0, c0v = p0;
-      fetch(e0, p0, a0, c0v);
-      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
-      while (b0 < n_batches) {
-        // ---- per-lane setup of one pair of batch b0 ----
+      // One batch: set up the current batch's pairs from their loaded end points and colours,
+      // issue the loads of the next batch and the pair entries of the one after, then walk.
+      auto step = [&](const uint32_t e_cur, const float4& p_cur, const float4& a_cur, const float4& c_cur,
+                      const uint32_t e_nxt, float4& p_nxt, float4& a_nxt, float4& c_nxt, uint32_t& b_aft,
+                      uint32_t& e_aft) {
+        // ---- per-lane setup of one pair of the current batch ----
         int packed = 0;  // n | q0 << 8 | xmajor << 16 | vertex colours << 17 | negative colour << 18
         float m_anchor = 0.f, slope = 0.f, s_anchor = 0.f, ds = 0.f;
         float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), dc = ca;
         int cq0 = 0, cq1 = 0, cq2 = 0, cq3 = 0;
-        if (e0 != kNoPair && (e0 >> 31) == (uint32_t)phase) {
+        if (e_cur != kNoPair && (e_cur >> 31) == (uint32_t)phase) {
           SegGeom g;
-          if (seg_geom(p0, v, g, phase)) {
+          if (seg_geom(p_cur, v, g, phase)) {
             const long maj0 = g.xmajor ? x0 : y0, min0 = g.xmajor ? y0 : x0;
             long a = g.klo > maj0 ? g.klo : maj0;
             long b = g.khi < maj0 + kTile - 1 ? g.khi : maj0 + kTile - 1;
@@ -637,8 +632,8 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
               // minor coordinate of the column's sample, tile-relative, minus the pixel-centre 0.5
               m_anchor = (float)(g.m0 + u0 * g.slope - (double)min0 - 0.5);
               slope = (float)g.slope;
-              ca = a0;
-              const float4 cb = c0v;
+              ca = a_cur;
+              const float4 cb = col1 ? c_cur : a_cur;
               dc = make_float4(cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w);
               const int vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
               s_anchor = (float)(u0 * g.inv_dM);
@@ -653,10 +648,9 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
           }
         }
         // ---- loads for the batches ahead ----
-        const uint32_t b2 = __builtin_amdgcn_readfirstlane(taken);
-        float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = p1, c1v = p1;
-        fetch(e1, p1, a1, c1v);
-        const uint32_t e2 = entry(b2);
+        fetch(e_nxt, p_nxt, a_nxt, c_nxt);
+        b_aft = __builtin_amdgcn_readfirstlane(taken);
+        e_aft = entry(b_aft);
         if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
         // ---- walk the pairs that have samples ----
         // Pair j's samples are columns jq0 .. jq0 + jn - 1 of the tile (jn <= 64): lane t takes
@@ -757,13 +751,29 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
             }
           }
         }
-        b0 = b1;
-        e0 = e1;
-        p0 = p1;
-        a0 = a1;
-        c0v = c1v;
-        b1 = b2;
-        e1 = e2;
+      };
+      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
+      uint32_t b0 = __builtin_amdgcn_readfirstlane(taken);
+      uint32_t e0 = entry(b0);
+      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
+      uint32_t b1 = __builtin_amdgcn_readfirstlane(taken);
+      uint32_t e1 = entry(b1);
+      float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), a0 = p0, c0v = p0, p1 = p0, a1 = p0, c1v = p0;
+      fetch(e0, p0, a0, c0v);
+      if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
+      // two batches per trip with the register sets swapped, so that every value loaded ahead is
+      // written once per trip and none is copied (a copy would wait for its load)
+      for (;;) {
+        if (b0 >= n_batches) break;
+        uint32_t b2, e2;
+        step(e0, p0, a0, c0v, e1, p1, a1, c1v, b2, e2);
+        if (b1 >= n_batches) break;
+        uint32_t b3, e3;
+        step(e1, p1, a1, c1v, e2, p0, a0, c0v, b3, e3);
+        b0 = b2;
+        e0 = e2;
+        b1 = b3;
+        e1 = e3;
       }
       __syncthreads();
       if (phase == 0) {
