@@ -542,20 +542,23 @@ __device__ __forceinline__ void transpose_tile(unsigned long long* t, int wave, 
 // addresses for every slope (a single padded layout serves one orientation's walks without
 // conflicts but stacks the lanes of the other's diagonal walks in a few banks).
 //
-// Sample positions are anchored on the tile column (the first major-axis pixel of the
-// tile), which is the same for every tile that shares a sample: m(q) = m_anchor + q * slope,
-// q = k - maj0, in FP32 relative to the tile's minor origin. Two tiles sharing a sample
-// differ only by the FP32 rounding of their anchors (< 1e-5 px).
+// A pair's sample positions are anchored on its first sample in the tile: m(k) = m_first +
+// (k - a) * slope in FP32 relative to the tile's minor origin, m_first rounded once from FP64.
+// Two tiles sharing a sample (its two deposits on either side of a tile edge) differ only by
+// FP32 rounding (< 1e-5 px).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __restrict__ descs,
                                                           const uint32_t* __restrict__ pairs,
                                                           const WorkItem* __restrict__ items,
                                                           const uint32_t* __restrict__ tile_split,
                                                           unsigned long long* split_fx, BatchCtl* ctl) {
-  __shared__ unsigned long long s_tile[4 * kPlane];
+  // one guard row, then the four planes
+  __shared__ unsigned long long s_tile[kTile + 4 * kPlane];
   __shared__ uint32_t s_item;
   __shared__ uint32_t s_batch[2];  // next batch of 64 pairs, per phase
+  unsigned long long* const tile = s_tile + kTile;
   const int lane = threadIdx.x & 63;
+  const float lane_f = (float)lane;
   const int wave = threadIdx.x >> 6;
   constexpr int kWaves = BLOCK / 64;
   if (ctl->flags & kCtlStop) return;  // a buffer overflowed: the host reruns the frame
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
     const uint32_t bc = __builtin_amdgcn_readfirstlane(D.blend_code);
     const uint32_t bc0 = bc & 15u, bc1 = (bc >> 4) & 15u, bc2 = (bc >> 8) & 15u, bc3 = (bc >> 12) & 15u;
     const float bk0 = D.blend_k[0], bk1 = D.blend_k[1], bk2 = D.blend_k[2], bk3 = D.blend_k[3];
-    for (int k = threadIdx.x; k < 4 * kPlane; k += BLOCK) s_tile[k] = 0ull;
+    for (int k = threadIdx.x; k < 4 * kPlane; k += BLOCK) tile[k] = 0ull;
     if (threadIdx.x == 0) s_batch[0] = s_batch[1] = 0u;
     __syncthreads();
     const int tx = (int)(W.tile % (uint32_t)v.tiles_x);
@@ -617,7 +620,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
                       uint32_t& e_aft) {
         // ---- per-lane setup of one pair of the current batch ----
         int packed = 0;  // n | q0 << 8 | xmajor << 16 | vertex colours << 17 | negative colour << 18
-        float m_anchor = 0.f, slope = 0.f, s_anchor = 0.f, ds = 0.f;
+        float m_first = 0.f, slope = 0.f, s_first = 0.f, ds = 0.f;
         float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), dc = ca;
         int cq0 = 0, cq1 = 0, cq2 = 0, cq3 = 0;
         if (e_cur != kNoPair && (e_cur >> 31) == (uint32_t)phase) {
@@ -628,15 +631,15 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
             long b = g.khi < maj0 + kTile - 1 ? g.khi : maj0 + kTile - 1;
             if (a <= b) minor_window(g, min0, min0 + kTile - 1, a, b);
             if (a <= b) {
-              const double u0 = ((double)maj0 + 0.5) - g.M0;
-              // minor coordinate of the column's sample, tile-relative, minus the pixel-centre 0.5
-              m_anchor = (float)(g.m0 + u0 * g.slope - (double)min0 - 0.5);
+              const double u0 = ((double)a + 0.5) - g.M0;
+              // minor coordinate of the first sample, tile-relative, minus the pixel-centre 0.5
+              m_first = (float)(g.m0 + u0 * g.slope - (double)min0 - 0.5);
               slope = (float)g.slope;
               ca = a_cur;
               const float4 cb = col1 ? c_cur : a_cur;
               dc = make_float4(cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w);
               const int vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
-              s_anchor = (float)(u0 * g.inv_dM);
+              s_first = (float)(u0 * g.inv_dM);
               ds = (float)g.inv_dM;
               cq0 = fx(blend_value(bc0, ca.x, ca.w, bk0), cs0);
               cq1 = fx(blend_value(bc1, ca.y, ca.w, bk1), cs1);
@@ -654,8 +657,9 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
         if (lane == 0) taken = atomicAdd(&s_batch[phase], 1u);
         // ---- walk the pairs that have samples ----
         // Pair j's samples are columns jq0 .. jq0 + jn - 1 of the tile (jn <= 64): lane t takes
-        // column jq0 + t. Deposits are exact 64-bit products: (2^20 - wq) * c at minor jl and
-        // wq * c at jl + 1, which add up to exactly c << 20.
+        // column jq0 + t at minor position m = m_first + t * slope. Deposits are exact 64-bit
+        // products: (2^20 - wq) * c at minor pixel jl and wq * c at jl + 1, which add up to
+        // exactly c << 20 (f < 1 keeps wq = trunc(f * 2^20 + 1/2) <= 2^20).
         uint64_t todo = __ballot(packed != 0);
         while (todo) {
           const int j = __builtin_ctzll(todo);
@@ -664,90 +668,72 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
           const int jn = jp & 0xff, jq0 = (jp >> 8) & 0xff, jvc = (jp >> 17) & 1;
           const int jneg = (jp >> 18) & 1;
           wave_samples += (unsigned long long)jn;
-          const float jm = bcast_f(m_anchor, j), jsl = bcast_f(slope, j);
-          const int q = jq0 + lane;
+          const float jm = bcast_f(m_first, j), jsl = bcast_f(slope, j);
+          // lane's sample: minor pixel jl, weights wl (pixel jl) and wq (jl + 1), and the entry
+          // t of (jl, jq0 + lane) in plane 0 (t2 in plane 2); the row below and the next plane
+          // are at fixed offsets from these. Plane 0 starts one guard row into s_tile, so row
+          // jl = -1 has an address; nothing is deposited there. Lanes past jn compute a finite
+          // position and deposit nothing.
+          const float m = __builtin_fmaf(lane_f, jsl, jm);
+          const int jl = (int)__builtin_floorf(m);
+          const int wq = (int)__builtin_fmaf(__builtin_amdgcn_fractf(m), (float)kWOne, 0.5f);
+          const int wl = kWOne - wq;
+          const int e0 = jl * kTile + lane + jq0;
+          // plane 2's entry as an opaque register: the deposits of planes 2 and 3 then address
+          // it with the instruction's offset field instead of each adding 64 KB and more
+          int e2 = e0 + 2 * kPlane;
+          asm volatile("" : "+v"(e2));
+          unsigned long long* const t = tile + e0;
+          unsigned long long* const t2 = tile + e2;
+          const bool lo_in = lane < jn && (unsigned)jl < (unsigned)kTile;
+          const bool hi_in = lane < jn && (unsigned)(jl + 1) < (unsigned)kTile;
           if (!jvc && !jneg) {
             // non-negative colour: unsigned 32x32 -> 64-bit products (one multiply per deposit)
             const unsigned jc0 = (unsigned)bcast_i(cq0, j), jc1 = (unsigned)bcast_i(cq1, j),
                            jc2 = (unsigned)bcast_i(cq2, j), jc3 = (unsigned)bcast_i(cq3, j);
-            if (lane < jn) {
-              const float m = __builtin_fmaf((float)q, jsl, jm);
-              const float jf = __builtin_floorf(m);
-              const int jl = (int)jf;
-              int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
-              wq = wq > kWOne ? kWOne : wq;
-              const unsigned uq = (unsigned)wq, ul = (unsigned)(kWOne - wq);
-              const int i1 = (jl + 1) * kTile + q;  // minor jl + 1, major q
-              const int i0 = i1 - kTile;
-              if ((unsigned)jl < (unsigned)kTile) {
-                atomicAdd(&s_tile[i0], (unsigned long long)ul * jc0);
-                atomicAdd(&s_tile[kPlane + i0], (unsigned long long)ul * jc1);
-                atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)ul * jc2);
-                atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)ul * jc3);
-              }
-              if ((unsigned)(jl + 1) < (unsigned)kTile) {
-                atomicAdd(&s_tile[i1], (unsigned long long)uq * jc0);
-                atomicAdd(&s_tile[kPlane + i1], (unsigned long long)uq * jc1);
-                atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)uq * jc2);
-                atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)uq * jc3);
-              }
+            const unsigned ul = (unsigned)wl, uq = (unsigned)wq;
+            if (lo_in) {
+              atomicAdd(t, (unsigned long long)ul * jc0);
+              atomicAdd(t + kPlane, (unsigned long long)ul * jc1);
+              atomicAdd(t2, (unsigned long long)ul * jc2);
+              atomicAdd(t2 + kPlane, (unsigned long long)ul * jc3);
             }
-          } else if (!jvc) {
-            const int jc0 = bcast_i(cq0, j), jc1 = bcast_i(cq1, j), jc2 = bcast_i(cq2, j), jc3 = bcast_i(cq3, j);
-            if (lane < jn) {
-              const float m = __builtin_fmaf((float)q, jsl, jm);
-              const float jf = __builtin_floorf(m);
-              const int jl = (int)jf;  // tile-relative minor index of the first deposit pixel
-              int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
-              wq = wq > kWOne ? kWOne : wq;
-              const int wl = kWOne - wq;
-              const int i1 = (jl + 1) * kTile + q;  // minor jl + 1, major q
-              const int i0 = i1 - kTile;
-              if ((unsigned)jl < (unsigned)kTile) {
-                atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * jc0));
-                atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * jc1));
-                atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * jc2));
-                atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * jc3));
-              }
-              if ((unsigned)(jl + 1) < (unsigned)kTile) {
-                atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * jc0));
-                atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * jc1));
-                atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * jc2));
-                atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * jc3));
-              }
+            if (hi_in) {
+              atomicAdd(t + kTile, (unsigned long long)uq * jc0);
+              atomicAdd(t + kTile + kPlane, (unsigned long long)uq * jc1);
+              atomicAdd(t2 + kTile, (unsigned long long)uq * jc2);
+              atomicAdd(t2 + kTile + kPlane, (unsigned long long)uq * jc3);
             }
           } else {
-            // per-vertex colours: the colour is interpolated along the segment at each sample
-            const float jds = bcast_f(ds, j), jsa = bcast_f(s_anchor, j);
-            const float a0 = bcast_f(ca.x, j), a1 = bcast_f(ca.y, j), a2 = bcast_f(ca.z, j), a3 = bcast_f(ca.w, j);
-            const float d0 = bcast_f(dc.x, j), d1 = bcast_f(dc.y, j), d2 = bcast_f(dc.z, j), d3 = bcast_f(dc.w, j);
-            if (lane < jn) {
-              const float m = __builtin_fmaf((float)q, jsl, jm);
-              const float jf = __builtin_floorf(m);
-              const int jl = (int)jf;
-              int wq = (int)__builtin_fmaf(m - jf, (float)kWOne, 0.5f);
-              wq = wq > kWOne ? kWOne : wq;
-              const int wl = kWOne - wq;
-              const float sp = __builtin_fmaf((float)q, jds, jsa);
+            int c0, c1, c2, c3;
+            if (!jvc) {
+              c0 = bcast_i(cq0, j);
+              c1 = bcast_i(cq1, j);
+              c2 = bcast_i(cq2, j);
+              c3 = bcast_i(cq3, j);
+            } else {
+              // per-vertex colours: the colour is interpolated along the segment at each sample
+              const float jds = bcast_f(ds, j), jsa = bcast_f(s_first, j);
+              const float a0 = bcast_f(ca.x, j), a1 = bcast_f(ca.y, j), a2 = bcast_f(ca.z, j), a3 = bcast_f(ca.w, j);
+              const float d0 = bcast_f(dc.x, j), d1 = bcast_f(dc.y, j), d2 = bcast_f(dc.z, j), d3 = bcast_f(dc.w, j);
+              const float sp = __builtin_fmaf(lane_f, jds, jsa);
               const float al = __builtin_fmaf(sp, d3, a3);
-              const int c0 = fx(blend_value(bc0, __builtin_fmaf(sp, d0, a0), al, bk0), cs0);
-              const int c1 = fx(blend_value(bc1, __builtin_fmaf(sp, d1, a1), al, bk1), cs1);
-              const int c2 = fx(blend_value(bc2, __builtin_fmaf(sp, d2, a2), al, bk2), cs2);
-              const int c3 = fx(blend_value(bc3, al, al, bk3), cs3);
-              const int i1 = (jl + 1) * kTile + q;  // minor jl + 1, major q
-              const int i0 = i1 - kTile;
-              if ((unsigned)jl < (unsigned)kTile) {
-                atomicAdd(&s_tile[i0], (unsigned long long)((long long)wl * c0));
-                atomicAdd(&s_tile[kPlane + i0], (unsigned long long)((long long)wl * c1));
-                atomicAdd(&s_tile[2 * kPlane + i0], (unsigned long long)((long long)wl * c2));
-                atomicAdd(&s_tile[3 * kPlane + i0], (unsigned long long)((long long)wl * c3));
-              }
-              if ((unsigned)(jl + 1) < (unsigned)kTile) {
-                atomicAdd(&s_tile[i1], (unsigned long long)((long long)wq * c0));
-                atomicAdd(&s_tile[kPlane + i1], (unsigned long long)((long long)wq * c1));
-                atomicAdd(&s_tile[2 * kPlane + i1], (unsigned long long)((long long)wq * c2));
-                atomicAdd(&s_tile[3 * kPlane + i1], (unsigned long long)((long long)wq * c3));
-              }
+              c0 = fx(blend_value(bc0, __builtin_fmaf(sp, d0, a0), al, bk0), cs0);
+              c1 = fx(blend_value(bc1, __builtin_fmaf(sp, d1, a1), al, bk1), cs1);
+              c2 = fx(blend_value(bc2, __builtin_fmaf(sp, d2, a2), al, bk2), cs2);
+              c3 = fx(blend_value(bc3, al, al, bk3), cs3);
+            }
+            if (lo_in) {
+              atomicAdd(t, (unsigned long long)((long long)wl * c0));
+              atomicAdd(t + kPlane, (unsigned long long)((long long)wl * c1));
+              atomicAdd(t2, (unsigned long long)((long long)wl * c2));
+              atomicAdd(t2 + kPlane, (unsigned long long)((long long)wl * c3));
+            }
+            if (hi_in) {
+              atomicAdd(t + kTile, (unsigned long long)((long long)wq * c0));
+              atomicAdd(t + kTile + kPlane, (unsigned long long)((long long)wq * c1));
+              atomicAdd(t2 + kTile, (unsigned long long)((long long)wq * c2));
+              atomicAdd(t2 + kTile + kPlane, (unsigned long long)((long long)wq * c3));
             }
           }
         }
@@ -777,7 +763,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
       }
       __syncthreads();
       if (phase == 0) {
-        transpose_tile<kWaves>(s_tile, wave, lane);
+        transpose_tile<kWaves>(tile, wave, lane);
         __syncthreads();
       }
     }
@@ -787,7 +773,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
       for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-          const unsigned long long val = s_tile[c * kPlane + k];
+          const unsigned long long val = tile[c * kPlane + k];
           if (val) atomicAdd(&fx[c * kTile * kTile + k], val);
         }
       }
@@ -802,10 +788,10 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
         const int idx = k;  // row-major
         float4* dst = accum + (size_t)gy * v.width + gx;
         float4 o = *dst;
-        o.x += (float)((double)(long long)s_tile[idx] * un0);
-        o.y += (float)((double)(long long)s_tile[kPlane + idx] * un1);
-        o.z += (float)((double)(long long)s_tile[2 * kPlane + idx] * un2);
-        o.w += (float)((double)(long long)s_tile[3 * kPlane + idx] * un3);
+        o.x += (float)((double)(long long)tile[idx] * un0);
+        o.y += (float)((double)(long long)tile[kPlane + idx] * un1);
+        o.z += (float)((double)(long long)tile[2 * kPlane + idx] * un2);
+        o.w += (float)((double)(long long)tile[3 * kPlane + idx] * un3);
         *dst = o;
       }
     }
