@@ -154,3 +154,13 @@ def test_render_plain_small(ctx, orc):
     e = orc.rel_l1(img.rgba(), ref)
     print("plain small image rel L1 %.3e" % e)
     assert e < 1e-3
+
+
+def test_segment_buffer_limit(ctx):
+    """A segment buffer holds at most 2^31 - 1 segments (pair entries keep a segment index
+    in 31 bits); the request fails before any allocation."""
+    import ctypes as C
+    lib = abi.load()
+    out = C.c_void_p()
+    assert lib.lg_segments_create(ctx.handle, 1 << 31, 0, C.byref(out)) == abi.LG_E_OOM
+    assert not out.value
