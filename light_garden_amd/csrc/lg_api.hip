@@ -388,13 +388,13 @@ static int bvh_build(std::vector<BvhNode>& nodes, const std::vector<int>& order,
   return idx;
 }
 
-// The scene's BVH: object boxes padded by 1e-5 (1 + |coordinates|), far more than the FP32
-// culling pass's error, leaves of kBvhLeaf objects in Morton order of their box centres.
-static std::vector<BvhNode> build_bvh(const std::vector<Rec<double>>& recs, const std::vector<double>& verts,
+// Object boxes padded by 1e-5 (1 + |coordinates|), far more than the FP32 culling pass's
+// error and than the rounding of an FP64 inside test: a point outside an object's padded box
+// is neither hit by a culled ray nor contained.
+static std::vector<Box2> object_boxes(const std::vector<Rec<double>>& recs, const std::vector<double>& verts,
                                       const std::vector<int32_t>& obj_rec) {
   const int n = (int)obj_rec.size();
   std::vector<Box2> boxes(n);
-  Box2 all = empty_box();
   for (int o = 0; o < n; o++) {
     const int top = obj_rec[o];
     Box2 b = empty_box();
@@ -412,8 +412,16 @@ static std::vector<BvhNode> build_bvh(const std::vector<Rec<double>>& recs, cons
     b.hi[0] += pad;
     b.hi[1] += pad;
     boxes[o] = b;
-    all.add(b);
   }
+  return boxes;
+}
+
+// The scene's BVH over the padded object boxes: leaves of kBvhLeaf objects in Morton order of
+// their box centres.
+static std::vector<BvhNode> build_bvh(const std::vector<Box2>& boxes, const std::vector<int32_t>& obj_rec) {
+  const int n = (int)obj_rec.size();
+  Box2 all = empty_box();
+  for (int o = 0; o < n; o++) all.add(boxes[o]);
   std::vector<uint32_t> code(n);
   const double sx = all.hi[0] > all.lo[0] ? 65535.0 / (all.hi[0] - all.lo[0]) : 0.0;
   const double sy = all.hi[1] > all.lo[1] ? 65535.0 / (all.hi[1] - all.lo[1]) : 0.0;
@@ -469,7 +477,13 @@ int compile_scene(lg_scene* sc, hipStream_t st) {
   if ((rc = upload(sc->obj_rec, obj_rec.data(), obj_rec.size(), st))) return rc;
   if ((rc = upload(sc->verts32, verts32.data(), verts32.size(), st))) return rc;
   if ((rc = upload(sc->verts64, verts.data(), verts.size(), st))) return rc;
-  const std::vector<BvhNode> bvh = build_bvh(recs, verts, obj_rec);
+  const std::vector<Box2> boxes = object_boxes(recs, verts, obj_rec);
+  std::vector<float4> box32(boxes.size());
+  for (size_t o = 0; o < boxes.size(); o++)
+    box32[o] = make_float4(round_down(boxes[o].lo[0]), round_down(boxes[o].lo[1]), round_up(boxes[o].hi[0]),
+                           round_up(boxes[o].hi[1]));
+  if ((rc = upload(sc->obj_box, box32.data(), box32.size(), st))) return rc;
+  const std::vector<BvhNode> bvh = build_bvh(boxes, obj_rec);
   if ((rc = upload(sc->bvh_dev, bvh.data(), bvh.size(), st))) return rc;
   sc->n_bvh = (int)bvh.size();
   // lights

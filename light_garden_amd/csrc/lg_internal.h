@@ -183,6 +183,7 @@ struct SceneDesc {
   const LG_G double* obj_n64;
   const LG_G uint32_t* obj_flags;
   const LG_G int32_t* obj_rec;
+  const LG_G float4* obj_box;      // per object: padded world box {lo.x, lo.y, hi.x, hi.y}
   const LG_G void* verts;          // polygon vertices in the culling precision
   const LG_G double* verts64;
   const LG_G DevLight* lights;

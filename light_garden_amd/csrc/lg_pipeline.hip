@@ -352,6 +352,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
       D.obj_n64 = gbl(sc->obj_n64.as<const double>());
       D.obj_flags = gbl(sc->obj_flags.as<const uint32_t>());
       D.obj_rec = gbl(sc->obj_rec.as<const int32_t>());
+      D.obj_box = gbl(sc->obj_box.as<const float4>());
       D.verts = gbl(L.fp64 ? sc->verts64.p : sc->verts32.p);
       D.verts64 = gbl(sc->verts64.as<const double>());
       D.lights = gbl(sc->lights_dev.as<const DevLight>());

@@ -988,10 +988,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       }
 
       bool emit = false, keep0 = false, keep1 = false;
-      double px = 0, py = 0;
+      double px = 0, py = 0, nx = 0, ny = 0;
       double d0x = 0, d0y = 0, d1x = 0, d1y = 0;
       float c0[4] = {0, 0, 0, 0}, c1[4] = {0, 0, 0, 0};
       double n_refr = ior;
+      bool hit = false, seek = false;  // seek: exiting a closed object, medium not yet found
+      int obj = -1;
+      uint32_t fl = 0;
+      double n2 = 1.0;
       if (active) {
         const RayT<double> rd{ox, oy, dx, dy};
         Nearest<double> h;
@@ -1013,53 +1017,64 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             px = ox + h.t * dx;
             py = oy + h.t * dy;
           }
-          double nx, ny;
           hit_normal<double>(R, verts64, h.code, h.s, px, py, nx, ny);
-          const int obj = R.obj;
-          const uint32_t fl = s_obj_flags[obj];
+          obj = R.obj;
+          fl = s_obj_flags[obj];
+          hit = true;
           emit = true;
           if (fl & OBJ_MATERIAL) {
-            const bool exiting = (fl & OBJ_CLOSED) && (dx * nx + dy * ny > 0.0);
-            double n2 = 1.0;
-            if (exiting) {
-              for (int o = 0; o < n_objs; o++) {
-                if (o == obj) continue;
-                if (!(s_obj_flags[o] & OBJ_MATERIAL)) continue;
-                if (contains_object<double>(recs64, verts64, s_obj_rec[o], px, py)) {
-                  n2 = gen(D.obj_n64)[o];
-                  break;
-                }
-              }
-            } else {
-              n2 = gen(D.obj_n64)[obj];
-            }
-            double Rf;
-            const bool has_refr = refract_dir<double>(dx, dy, nx, ny, ior, n2, d0x, d0y, d1x, d1y, Rf);
-            const float rf = (float)Rf;       // tracer.rs:454
-            const float omr = 1.0f - rf;      // tracer.rs:455
-            c0[0] = c[0] * rf; c0[1] = c[1] * rf; c0[2] = c[2] * rf; c0[3] = c[3];
-            keep0 = !below_cutoff(c0, D.cutoff);
-            if (has_refr) {
-              c1[0] = c[0] * omr; c1[1] = c[1] * omr; c1[2] = c[2] * omr; c1[3] = c[3];
-              keep1 = !below_cutoff(c1, D.cutoff);
-              n_refr = n2;
-            }
-          } else {
-            // Ray::reflect (tracer.rs:476-480)
-            const double dn = dx * nx + dy * ny;
-            d0x = dx - 2.0 * dn * nx;
-            d0y = dy - 2.0 * dn * ny;
-            normalize_v(d0x, d0y);
-            c0[0] = c[0]; c0[1] = c[1]; c0[2] = c[2]; c0[3] = c[3];
-            keep0 = true;  // same colour as the parent, which passed the cutoff
+            seek = (fl & OBJ_CLOSED) && (dx * nx + dy * ny > 0.0);
+            if (!seek) n2 = gen(D.obj_n64)[obj];
           }
         } else {
           emit = canvas_exit<double>(rd, D.canvas, px, py);
         }
-        if (last_gen) {
-          keep0 = false;
-          keep1 = false;
+      }
+      // The medium a ray exits into: the first other material object, in scene order, that
+      // contains the exit point (tracer.rs:431-440), else air. The loop over objects is
+      // uniform across the wave (its flags and padded boxes are scalar loads); a lane runs the
+      // exact FP64 inside test only for objects whose box holds its point, and stops at the
+      // first that contains it.
+      {
+        const float4* obj_box = gen(D.obj_box);
+        uint64_t seeking = __ballot(seek);
+        for (int o = 0; seeking && o < n_objs; o++) {
+          if (!(s_obj_flags[o] & OBJ_MATERIAL)) continue;
+          const float4 bx = obj_box[o];
+          if (seek && o != obj && px >= (double)bx.x && px <= (double)bx.z && py >= (double)bx.y &&
+              py <= (double)bx.w && contains_object<double>(recs64, verts64, s_obj_rec[o], px, py)) {
+            n2 = gen(D.obj_n64)[o];
+            seek = false;
+          }
+          seeking = __ballot(seek);
         }
+      }
+      if (hit) {
+        if (fl & OBJ_MATERIAL) {
+          double Rf;
+          const bool has_refr = refract_dir<double>(dx, dy, nx, ny, ior, n2, d0x, d0y, d1x, d1y, Rf);
+          const float rf = (float)Rf;       // tracer.rs:454
+          const float omr = 1.0f - rf;      // tracer.rs:455
+          c0[0] = c[0] * rf; c0[1] = c[1] * rf; c0[2] = c[2] * rf; c0[3] = c[3];
+          keep0 = !below_cutoff(c0, D.cutoff);
+          if (has_refr) {
+            c1[0] = c[0] * omr; c1[1] = c[1] * omr; c1[2] = c[2] * omr; c1[3] = c[3];
+            keep1 = !below_cutoff(c1, D.cutoff);
+            n_refr = n2;
+          }
+        } else {
+          // Ray::reflect (tracer.rs:476-480)
+          const double dn = dx * nx + dy * ny;
+          d0x = dx - 2.0 * dn * nx;
+          d0y = dy - 2.0 * dn * ny;
+          normalize_v(d0x, d0y);
+          c0[0] = c[0]; c0[1] = c[1]; c0[2] = c[2]; c0[3] = c[3];
+          keep0 = true;  // same colour as the parent, which passed the cutoff
+        }
+      }
+      if (last_gen) {
+        keep0 = false;
+        keep1 = false;
       }
 
       // ---- workgroup-aggregated append ----
