@@ -16,10 +16,13 @@
 
 namespace lg {
 
-constexpr size_t kCounterStride = 128;  // per scene: u64 segments, then u32 per generation
+// per scene: the u64 segment count, and one cache line further the u32 children per
+// generation (the trace's appends add to both; apart, they do not contend for one L2 line)
+constexpr size_t kCounterStride = 256;
+constexpr size_t kGenCountOffset = 128;
 constexpr uint32_t kChunkMin = 256;     // segments per bin chunk, at least
 constexpr uint32_t kBinWaves = 4;       // bin workgroups per compute unit (bin_target = 2 per workgroup)
-static_assert(8 + 4 * kMaxGen <= kCounterStride, "counter stride");
+static_assert(kGenCountOffset + 4 * kMaxGen <= kCounterStride, "counter stride");
 // Pairs per tile work item: total / (2 * n_cu) + 1, clamped to [min_piece, max_piece]. A tile
 // is split when it holds more than one piece. The merge buffer holds a slot per split tile:
 // it starts at 2 * n_cu + 64 slots and grows to what frames report (overflow -> rerun).
@@ -343,7 +346,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
     SceneDesc& D = hd[i];
     std::memset(&D, 0, sizeof(D));
     D.seg_count = gbl(reinterpret_cast<unsigned long long*>(P.counters.as<char>() + i * kCounterStride));
-    D.gen_count = gbl(reinterpret_cast<uint32_t*>(P.counters.as<char>() + i * kCounterStride + 8));
+    D.gen_count = gbl(reinterpret_cast<uint32_t*>(P.counters.as<char>() + i * kCounterStride + kGenCountOffset));
     float max_abs = 0.0f;
     if (trace) {
       lg_scene* sc = e.scene;
@@ -474,7 +477,7 @@ static int batch_finish(lg_ctx* ctx, Pipeline& P, bool profile, Batch& B, bool* 
   for (int i = 0; trace && i < n; i++) {
     const Entry& e = E[i];
     const uint64_t segs = *reinterpret_cast<const uint64_t*>(out + i * kCounterStride);
-    const uint32_t* gc = reinterpret_cast<const uint32_t*>(out + i * kCounterStride + 8);
+    const uint32_t* gc = reinterpret_cast<const uint32_t*>(out + i * kCounterStride + kGenCountOffset);
     uint64_t qmax = 0;
     for (uint32_t g = 0; g < e.params.max_bounce; g++) qmax = std::max<uint64_t>(qmax, gc[g]);
     if (segs > e.segs->cap) {
