@@ -163,25 +163,30 @@ def bench_batched(args):
         trs.append(tr)
         segs.append(lg.Segments(ctx))
         imgs.append(lg.Image(ctx, sc.width, sc.height, sc.canvas))
-    tuned = autotune.apply(ctx, "batched_uneven") if not args.no_tuning_cache else None
-    tuning_source = "tuning_mi355x.json" if tuned else "library defaults"
+    log = lambda m: print("autotune: " + m, file=sys.stderr, flush=True)
+    tune_out = os.path.join(ROOT, "gpurun_out")
 
-    def batch_frame():
-        lg.batch_render(ctx, trs, segs, imgs)
-
-    if args.autotune:
-        best, table = autotune.tune(ctx, batch_frame, repeats=2,
-                                    log=lambda m: print("autotune: " + m, file=sys.stderr, flush=True))
-        tuning_source = "autotuned in this run"
-        out = os.path.join(ROOT, "gpurun_out", "tuning_batched_uneven.json")
-        os.makedirs(os.path.dirname(out), exist_ok=True)
-        autotune.save("batched_uneven", best, min(r["ms"] for r in table), out,
-                      note="coordinate descent over %d settings" % len(table))
-
-    # ---- plain per-scene path: one lg_render per scene, eager launches ----
+    # ---- plain per-scene path: one lg_render per scene, eager launches, under the launch
+    # tuning that is best for it (tuned on its own, not the batch's) ----
     ctx.set_graphs(False)
-    for tr, sg, im in zip(trs, segs, imgs):  # warm-up: capacities settle
-        tr.render(sg, im)
+    plain_tuned = autotune.apply(ctx, "per_scene_uneven") if not args.no_tuning_cache else None
+    plain_source = "tuning_mi355x.json" if plain_tuned else "library defaults"
+
+    def plain_pass():
+        for tr, sg, im in zip(trs, segs, imgs):
+            tr.render(sg, im)
+
+    plain_pass()  # warm-up: capacities settle
+    ctx.synchronize()
+    if args.autotune:
+        best, table = autotune.tune(ctx, plain_pass, repeats=2, log=log)
+        plain_source = "autotuned in this run"
+        os.makedirs(tune_out, exist_ok=True)
+        autotune.save("per_scene_uneven", best, min(r["ms"] for r in table),
+                      os.path.join(tune_out, "tuning_per_scene_uneven.json"),
+                      note="coordinate descent over %d settings, one lg_render per scene" % len(table))
+    plain_tuning = ctx.tuning()
+    plain_pass()
     ctx.synchronize()
     n_seg = sum(s.count() for s in segs)
     plain_ref = [im.rgba16f().copy() for im in imgs]
@@ -196,6 +201,23 @@ def bench_batched(args):
     ctx.synchronize()
     plain_s = (time.perf_counter() - t0) / args.steps
     ctx.set_graphs(True)
+
+    # ---- the batch's own tuning ----
+    tuned = autotune.apply(ctx, "batched_uneven") if not args.no_tuning_cache else None
+    tuning_source = "tuning_mi355x.json" if tuned else "library defaults"
+
+    def batch_frame():
+        lg.batch_render(ctx, trs, segs, imgs)
+
+    if args.autotune:
+        batch_frame()
+        ctx.synchronize()
+        best, table = autotune.tune(ctx, batch_frame, repeats=2, log=log)
+        tuning_source = "autotuned in this run"
+        os.makedirs(tune_out, exist_ok=True)
+        autotune.save("batched_uneven", best, min(r["ms"] for r in table),
+                      os.path.join(tune_out, "tuning_batched_uneven.json"),
+                      note="coordinate descent over %d settings" % len(table))
 
     # ---- batched: every scene in the same launches ----
     for _ in range(args.warmup):
@@ -244,6 +266,7 @@ def bench_batched(args):
                          "latency_ms": {"p50": percentile(lat_i, 50), "p99": percentile(lat_i, 99)}},
         "plain_per_scene": {"value": n_seg / plain_s, "ms_per_pass": plain_s * 1e3,
                             "scene_latency_ms": {"p50": percentile(per_scene, 50), "p99": percentile(per_scene, 99)},
+                            "tuning": plain_tuning, "tuning_source": plain_source,
                             "note": "one lg_render per scene, eager launches (per-scene graphs would be "
                                     "recaptured for every scene: 256 shapes, 8 cached)"},
         "batched_vs_plain": plain_s / batch_s,
@@ -286,13 +309,20 @@ def bench_serving(args):
     t_batch = time.perf_counter() - t0
     seg_per_scene = [s.count() for s in segs]
     cap_rps = n / t_batch
+    # the per-scene rate under the per-scene path's own tuning (after a warm-up pass)
     ctx.set_graphs(False)
+    batch_tuning = ctx.tuning()
+    autotune.apply(ctx, "per_scene_uneven")
+    for tr, sg, im in zip(trs, segs, imgs):
+        tr.render(sg, im)
+    ctx.synchronize()
     t0 = time.perf_counter()
     for tr, sg, im in zip(trs, segs, imgs):
         tr.render(sg, im)
     ctx.synchronize()
     t_plain = time.perf_counter() - t0
     ctx.set_graphs(True)
+    ctx.set_tuning(**batch_tuning)
     print("serving: batched capacity %.1f scenes/s (%.1f ms per %d scenes), plain %.1f scenes/s"
           % (cap_rps, t_batch * 1e3, n, n / t_plain), file=sys.stderr, flush=True)
     for s in segs:
