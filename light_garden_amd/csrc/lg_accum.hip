@@ -314,13 +314,20 @@ __global__ void tile_starts_kernel(const SceneDesc* __restrict__ descs, const ui
   }
 }
 
-// pieces[g] = number of work items of tile g; counts the items of every size class.
+// pieces[g] = number of work items of tile g; counts the items of every size class. With
+// `whole_images` (the frame clears, accumulates and resolves every image) a tile without
+// pairs gets one empty item (size class 0), which writes its cleared pixels.
 __global__ void tile_pieces_kernel(const uint32_t* __restrict__ tile_start, int n_tiles_total, BatchCtl* ctl,
-                                   uint32_t* pieces) {
+                                   uint32_t* pieces, uint32_t whole_images) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_tiles_total) return;
   const uint32_t piece = ctl->piece;
   const uint32_t cnt = tile_start[g + 1] - tile_start[g];
+  if (cnt == 0) {
+    pieces[g] = whole_images ? 1u : 0u;
+    if (whole_images) atomicAdd(&ctl->class_count[0], 1u);
+    return;
+  }
   const uint32_t full = cnt / piece, rem = cnt - full * piece;
   pieces[g] = full + (rem ? 1u : 0u);
   if (full) atomicAdd(&ctl->class_count[31 - __clz(piece)], full);
@@ -367,7 +374,7 @@ __global__ void write_items_kernel(const SceneDesc* __restrict__ descs, const ui
     w.tile = t;
     w.begin = b + p * piece;
     w.end = (w.begin + piece < e) ? w.begin + piece : e;
-    const int cls = 31 - __clz(w.end - w.begin);
+    const int cls = w.end > w.begin ? 31 - __clz(w.end - w.begin) : 0;
     const uint32_t slot = ctl->class_base[cls] + atomicAdd(&ctl->class_cursor[cls], 1u);
     if (slot < items_cap) items[slot] = w;
   }
@@ -508,6 +515,17 @@ __device__ __forceinline__ double bcast_d(double v, int lane) {
 
 __device__ __forceinline__ int fx(float v, float scale) { return (int)__builtin_rintf(v * scale); }
 
+// FP32 accumulator -> RGBA16F, round to nearest even, alpha + 1 (the target's clear alpha,
+// renderer.rs:175). The one expression of the resolve pass and the fused flushes.
+__device__ __forceinline__ ushort4 resolve_px(const float4 a) {
+  ushort4 o;
+  o.x = __builtin_bit_cast(unsigned short, (_Float16)a.x);
+  o.y = __builtin_bit_cast(unsigned short, (_Float16)a.y);
+  o.z = __builtin_bit_cast(unsigned short, (_Float16)a.z);
+  o.w = __builtin_bit_cast(unsigned short, (_Float16)(a.w + 1.0f));
+  return o;
+}
+
 // In-place transpose of the four 64x64 planes, (r, c) <-> (c, r). Job j of 128 is plane j / 32
 // and diagonal pair d = j % 32 + 1: lanes 0 .. 63 - d swap the elements of diagonal c - r = d,
 // lanes 64 - d .. 63 those of diagonal 64 - d (d < 32; diagonal 32 takes lanes 0 .. 31 alone).
@@ -587,12 +605,28 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
     const uint32_t bc = __builtin_amdgcn_readfirstlane(D.blend_code);
     const uint32_t bc0 = bc & 15u, bc1 = (bc >> 4) & 15u, bc2 = (bc >> 8) & 15u, bc3 = (bc >> 12) & 15u;
     const float bk0 = D.blend_k[0], bk1 = D.blend_k[1], bk2 = D.blend_k[2], bk3 = D.blend_k[3];
-    for (int k = threadIdx.x; k < 4 * kPlane; k += BLOCK) tile[k] = 0ull;
-    if (threadIdx.x == 0) s_batch[0] = s_batch[1] = 0u;
-    __syncthreads();
     const int tx = (int)(W.tile % (uint32_t)v.tiles_x);
     const int ty = (int)(W.tile / (uint32_t)v.tiles_x);
     const int x0 = tx * kTile, y0 = ty * kTile;
+    // the frame clears and resolves the image here: the flush writes the accumulator and the
+    // RGBA16F target instead of adding to a cleared accumulator
+    const bool whole = (D.flags & SCENE_FRESH_IMAGE) != 0;
+    if (W.begin == W.end) {  // a tile without pairs in such a frame: its cleared pixels
+      ushort4* out16 = gen(D.out16);
+      for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
+        const int gx = x0 + (k % kTile), gy = y0 + (k / kTile);
+        if (gx >= v.width || gy >= v.height) continue;
+        const size_t px = (size_t)gy * v.width + gx;
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        accum[px] = z;
+        out16[px] = resolve_px(z);
+      }
+      __syncthreads();
+      continue;
+    }
+    for (int k = threadIdx.x; k < 4 * kPlane; k += BLOCK) tile[k] = 0ull;
+    if (threadIdx.x == 0) s_batch[0] = s_batch[1] = 0u;
+    __syncthreads();
     const uint32_t n_batches = (W.end - W.begin + 63u) >> 6;
     // pair entry of this lane in batch b, kNoPair past the item's end
     auto entry = [&](uint32_t b) -> uint32_t {
@@ -786,13 +820,14 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
         int gx = x0 + cc, gy = y0 + r;
         if (gx >= v.width || gy >= v.height) continue;
         const int idx = k;  // row-major
-        float4* dst = accum + (size_t)gy * v.width + gx;
-        float4 o = *dst;
+        const size_t px = (size_t)gy * v.width + gx;
+        float4 o = whole ? make_float4(0.f, 0.f, 0.f, 0.f) : accum[px];
         o.x += (float)((double)(long long)tile[idx] * un0);
         o.y += (float)((double)(long long)tile[kPlane + idx] * un1);
         o.z += (float)((double)(long long)tile[2 * kPlane + idx] * un2);
         o.w += (float)((double)(long long)tile[3 * kPlane + idx] * un3);
-        *dst = o;
+        accum[px] = o;
+        if (whole) gen(D.out16)[px] = resolve_px(o);
       }
     }
     __syncthreads();
@@ -819,24 +854,26 @@ __global__ __launch_bounds__(256) void tile_merge_kernel(const SceneDesc* __rest
     double un[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) un[c] = __builtin_ldexp(1.0, -(scene_fixed_bits(D, c) + kWBits));
+    const bool whole = (D.flags & SCENE_FRESH_IMAGE) != 0;  // the frame clears and resolves here
     unsigned long long* fx = split_fx + (size_t)sl * kSlotU64;
     for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
       unsigned long long a[4];
 #pragma unroll
       for (int c = 0; c < 4; c++) a[c] = fx[c * kTile * kTile + k];
-      if ((a[0] | a[1] | a[2] | a[3]) == 0ull) continue;
+      if ((a[0] | a[1] | a[2] | a[3]) == 0ull && !whole) continue;
 #pragma unroll
       for (int c = 0; c < 4; c++)
         if (a[c]) fx[c * kTile * kTile + k] = 0ull;
       const int gx = x0 + (k % kTile), gy = y0 + (k / kTile);
       if (gx >= v.width || gy >= v.height) continue;
-      float4* dst = gen(D.accum) + (size_t)gy * v.width + gx;
-      float4 o = *dst;
+      const size_t px = (size_t)gy * v.width + gx;
+      float4 o = whole ? make_float4(0.f, 0.f, 0.f, 0.f) : gen(D.accum)[px];
       o.x += (float)((double)(long long)a[0] * un[0]);
       o.y += (float)((double)(long long)a[1] * un[1]);
       o.z += (float)((double)(long long)a[2] * un[2]);
       o.w += (float)((double)(long long)a[3] * un[3]);
-      *dst = o;
+      gen(D.accum)[px] = o;
+      if (whole) gen(D.out16)[px] = resolve_px(o);
     }
   }
 }
@@ -854,14 +891,7 @@ __global__ void pixels_kernel(const SceneDesc* __restrict__ descs, const uint64_
   if (MODE == 0) {
     gen(D.accum)[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   } else {
-    const float4 a = gen(D.accum)[p];
-    const _Float16 r = (_Float16)a.x, g = (_Float16)a.y, b = (_Float16)a.z, al = (_Float16)(a.w + 1.0f);
-    ushort4 o;
-    o.x = __builtin_bit_cast(unsigned short, r);
-    o.y = __builtin_bit_cast(unsigned short, g);
-    o.z = __builtin_bit_cast(unsigned short, b);
-    o.w = __builtin_bit_cast(unsigned short, al);
-    gen(D.out16)[p] = o;
+    gen(D.out16)[p] = resolve_px(gen(D.accum)[p]);
   }
 }
 
@@ -942,14 +972,14 @@ hipError_t launch_work_items(const SceneDesc* descs, const uint32_t* tile_scene,
                              BatchCtl* ctl, uint32_t* tile_start, uint32_t* pieces,
                              WorkItem* items, uint32_t items_cap, uint32_t* tile_split, uint32_t* split_tiles,
                              uint32_t split_cap, uint64_t pairs_cap, uint32_t min_piece, uint32_t max_piece,
-                             uint32_t grid, hipStream_t st) {
+                             uint32_t grid, bool whole_images, hipStream_t st) {
   int n = n_tiles_total + 1;
   hipLaunchKernelGGL(tile_starts_kernel, dim3((n + 255) / 256), dim3(256), 0, st, descs, tile_scene, n_tiles_total,
                      chunk_pre, cbase, offsets, ctl, tile_start, pairs_cap, min_piece, max_piece, grid);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(tile_pieces_kernel, dim3((n_tiles_total + 255) / 256), dim3(256), 0, st, tile_start,
-                     n_tiles_total, ctl, pieces);
+                     n_tiles_total, ctl, pieces, whole_images ? 1u : 0u);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(class_bases_kernel, dim3(1), dim3(64), 0, st, ctl, items_cap);
   if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -97,7 +97,7 @@ hipError_t launch_work_items(const SceneDesc* descs, const uint32_t* tile_scene,
                              BatchCtl* ctl, uint32_t* tile_start, uint32_t* pieces,
                              WorkItem* items, uint32_t items_cap, uint32_t* tile_split, uint32_t* split_tiles,
                              uint32_t split_cap, uint64_t pairs_cap, uint32_t min_piece, uint32_t max_piece,
-                             uint32_t grid, hipStream_t st);
+                             uint32_t grid, bool whole_images, hipStream_t st);
 hipError_t launch_tile_accum(const SceneDesc* descs, const uint32_t* pairs, const WorkItem* items,
                              const uint32_t* tile_split, unsigned long long* split_fx, BatchCtl* ctl, int grid,
                              hipStream_t st);
@@ -178,6 +178,11 @@ struct lg_image {
 namespace lg {
 
 enum Stage : uint32_t { ST_CLEAR = 1u, ST_TRACE = 2u, ST_ACCUM = 4u, ST_RESOLVE = 8u };
+// The frame clears, accumulates into and resolves every image (lg_render, lg_batch_render,
+// the server): clear and resolve fold into the tile flushes.
+inline bool whole_images(uint32_t stages) {
+  return (stages & (ST_CLEAR | ST_ACCUM | ST_RESOLVE)) == (ST_CLEAR | ST_ACCUM | ST_RESOLVE);
+}
 
 struct Entry {
   lg_scene* scene = nullptr;

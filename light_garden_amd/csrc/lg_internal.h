@@ -158,7 +158,9 @@ struct AccumView {
 };
 
 enum SceneFlags : uint32_t {
-  SCENE_FIXED_COUNT = 1u      // accumulate `seg_fixed` uploaded segments (no trace)
+  SCENE_FIXED_COUNT = 1u,     // accumulate `seg_fixed` uploaded segments (no trace)
+  SCENE_FRESH_IMAGE = 2u      // the frame clears, accumulates and resolves the image: the tile
+                              // flushes write the accumulator and the RGBA16F target directly
 };
 
 // Device pointers kept in descriptors carry the global address space in their type: kernels

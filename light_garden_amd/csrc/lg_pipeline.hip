@@ -108,6 +108,7 @@ struct Scope {
 static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const std::vector<Entry>& E,
                    const Layout& L, uint32_t stages) {
   const int n = L.n;
+  const bool whole = whole_images(stages);
   SceneDesc* d_descs = P.descs.as<SceneDesc>();
   BatchCtl* d_ctl = P.ctl.as<BatchCtl>();
   const size_t desc_bytes = n * sizeof(SceneDesc);
@@ -119,7 +120,7 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const
   LG_HIP(hipMemcpyAsync(P.px_pre.p, in + desc_bytes + tile_bytes, px_bytes, hipMemcpyHostToDevice, st));
   LG_HIP(launch_fill_u32(P.counters.as<uint32_t>(), n * kCounterStride / 4, 0u, st));
   LG_HIP(launch_fill_u32(P.ctl.as<uint32_t>(), sizeof(BatchCtl) / 4, 0u, st));
-  if (stages & ST_CLEAR) {
+  if ((stages & ST_CLEAR) && !whole) {
     Scope s(ctx, profile, "clear", L.n_px);
     LG_HIP(launch_pixels(0, d_descs, P.px_pre.as<uint64_t>(), n, L.n_px, st));
   }
@@ -159,7 +160,8 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const
                                P.cbase.as<uint32_t>(), P.offsets.as<uint32_t>(), d_ctl,
                                P.tile_start.as<uint32_t>(), P.pieces.as<uint32_t>(), P.items.as<WorkItem>(),
                                P.items_cap, P.tile_split.as<uint32_t>(), P.split_tiles.as<uint32_t>(), P.split_cap,
-                               P.pairs_cap, ctx->tune.min_piece, ctx->tune.max_piece, (uint32_t)ctx->n_cu, st));
+                               P.pairs_cap, ctx->tune.min_piece, ctx->tune.max_piece, (uint32_t)ctx->n_cu, whole,
+                               st));
     }
     {
       Scope s(ctx, profile, "bin_fill", 0);
@@ -177,7 +179,7 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const
                                P.split_fx.as<unsigned long long>(), P.split_cap, ctx->n_cu, st));
     }
   }
-  if (stages & ST_RESOLVE) {
+  if ((stages & ST_RESOLVE) && !whole) {
     Scope s(ctx, profile, "resolve", L.n_px);
     LG_HIP(launch_pixels(1, d_descs, P.px_pre.as<uint64_t>(), n, L.n_px, st));
   }
@@ -389,6 +391,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
       }
     }
     if (e.img) {
+      if (whole_images(stages)) D.flags |= SCENE_FRESH_IMAGE;
       D.view = make_view(e.img->view);
       D.accum = gbl(e.img->accum.as<float4>());
       D.out16 = gbl(e.img->out16.as<ushort4>());

@@ -164,3 +164,47 @@ def test_segment_buffer_limit(ctx):
     out = C.c_void_p()
     assert lib.lg_segments_create(ctx.handle, 1 << 31, 0, C.byref(out)) == abi.LG_E_OOM
     assert not out.value
+
+
+@pytest.mark.parametrize("pieces", [None, (64, 64)], ids=["default_pieces", "split_tiles"])
+def test_whole_frame_equals_separate_stages(ctx, pieces):
+    """lg_render folds the clear and the resolve into the tile flushes (empty tiles get an item
+    of their own, split tiles resolve in the merge): the image and the accumulator equal, bit
+    for bit, those of lg_trace + clear + accumulate + resolve, also over a stale image."""
+    before = ctx.tuning()
+    try:
+        if pieces:
+            ctx.set_tuning(min_piece=pieces[0], max_piece=pieces[1])
+        scene = configs.large_single(num_rays=20000, width=320)
+        tr = lg.Tracer(ctx, scene.canvas)
+        for o in scene.objects:
+            tr.push_object(o)
+        for l in scene.lights:
+            tr.push_light(l)
+        tr.max_bounce = scene.max_bounce
+        segs = lg.Segments(ctx)
+        # the view is three times the canvas, so the outer tiles are empty; the size is not a
+        # multiple of the tile
+        view = (-3.0, 3.0, -3.0, 3.0)
+        img = lg.Image(ctx, 320, 272, view)
+        stale = lg.Image(ctx, 320, 272, view)
+        for im in (img, stale):
+            tr.render(segs, im)
+        tr.render(segs, stale)  # a second whole frame over the first
+        sep_segs = tr.trace()
+        sep = lg.Image(ctx, 320, 272, view)
+        sep.accumulate(sep_segs)
+        sep.accumulate(sep_segs)  # leave junk that the clear must remove
+        sep.clear()
+        sep.accumulate(sep_segs)
+        sep.resolve()
+        assert sep_segs.count() == segs.count() > 1000
+        for im in (img, stale):
+            np.testing.assert_array_equal(im.rgba16f(), sep.rgba16f())
+            np.testing.assert_array_equal(im.accum(), sep.accum())
+        a = sep.accum()
+        assert (a[..., :3] == 0).all(axis=-1).mean() > 0.5  # the empty-tile path ran
+        for x in (segs, sep_segs, img, stale, sep, tr):
+            x.close()
+    finally:
+        ctx.set_tuning(**before)
