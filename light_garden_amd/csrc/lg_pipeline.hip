@@ -105,8 +105,7 @@ struct Scope {
   ~Scope() { delete s; }
 };
 
-static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const std::vector<Entry>& E,
-                   const Layout& L, uint32_t stages) {
+static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const Layout& L, uint32_t stages) {
   const int n = L.n;
   const bool whole = whole_images(stages);
   SceneDesc* d_descs = P.descs.as<SceneDesc>();
@@ -428,7 +427,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
     if (!G) {
       P.make_room();
       LG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      rc = enqueue(ctx, P, st, profile, E, L, stages);
+      rc = enqueue(ctx, P, st, profile, L, stages);
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(st, &g);
       if (rc) {
@@ -450,7 +449,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
     G->used = ++P.tick;
     LG_HIP(hipGraphLaunch(G->exec, st));
   } else {
-    if ((rc = enqueue(ctx, P, st, profile, E, L, stages))) return rc;
+    if ((rc = enqueue(ctx, P, st, profile, L, stages))) return rc;
   }
   return LG_OK;
 }

@@ -235,6 +235,7 @@ __device__ inline bool crosses(const double4 A, const double4 B) {
   const double d = fabs(denom);
   const double t = neg ? -tn : tn, u = neg ? -un : un;
   const double two_d = d + d, eps = d * 0x1p-53;
+  // `&` on the conditions rather than `&&`: every term is evaluated, without branches
   const bool ok = (d >= 0x1p-500) & (d <= 0x1p500) & in_range(fabs(t)) & in_range(fabs(u));
   bool hit = ok & unit_fast(t, d, two_d, eps) & unit_fast(u, d, two_d, eps);
   if (__builtin_expect(!ok & (denom != 0.0), 0)) {
