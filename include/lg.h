@@ -59,7 +59,7 @@ enum lg_shape_kind {
 
 enum lg_logic_op { LG_LOGIC_AND = 0, LG_LOGIC_OR = 1, LG_LOGIC_AND_NOT = 2 };
 
-#define LG_MAX_SHAPES 7          /* nodes of one object's Logic tree (depth <= 2)    */
+#define LG_MAX_SHAPES 7          /* nodes of one object's Logic tree (<= 4 leaves)    */
 #define LG_MAX_POLY_VERTICES 16  /* vertices of all polygons of one object            */
 
 /*
@@ -74,7 +74,8 @@ enum lg_logic_op { LG_LOGIC_AND = 0, LG_LOGIC_OR = 1, LG_LOGIC_AND_NOT = 2 };
  *   CONVEX_POLYGON origin, rotation; vertices (local, counter-clockwise) at
  *                  lg_object.vertices[2*vertex_offset ..], n_vertices of them
  *   LOGIC          origin, rotation = local frame of the two operands; op,
- *                  child_a / child_b = node indices (> this node's index)
+ *                  child_a / child_b = node indices (> this node's index); an operand
+ *                  is a closed primitive (circle, rect, ellipse, polygon) or LOGIC
  */
 typedef struct lg_shape {
   int32_t kind;

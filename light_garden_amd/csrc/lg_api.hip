@@ -248,6 +248,38 @@ static int compile_prim(const lg_object& obj, int oi, const lg_shape& s, const F
   return LG_OK;
 }
 
+// Node ni of obj's shape tree and its subtree, in prefix order, in frame f (a Logic node's
+// origin and rotation compose with its parent's). Children come after their parent in
+// shapes[], so the tree has no cycles; leaves are closed primitives.
+static int compile_node(const lg_object& obj, int oi, int ni, const Frame& f, std::vector<Rec<double>>& recs,
+                        std::vector<double>& verts, std::vector<int>& leaves) {
+  const lg_shape& s = obj.shapes[ni];
+  if (s.kind != LG_SHAPE_LOGIC) {
+    if (!is_closed_prim(s.kind))
+      return fail(LG_E_UNSUPPORTED, "logic: operands must be circles, rects, ellipses, convex polygons or logic");
+    leaves.push_back((int)recs.size());
+    return compile_prim(obj, oi, s, f, recs, verts);
+  }
+  if (s.op < LG_LOGIC_AND || s.op > LG_LOGIC_AND_NOT) return fail(LG_E_INVALID, "logic: bad op");
+  if (s.child_a <= ni || s.child_b <= ni || s.child_a >= obj.n_shapes || s.child_b >= obj.n_shapes ||
+      s.child_a == s.child_b)
+    return fail(LG_E_INVALID, "logic: bad operand index (operands follow their node in shapes[])");
+  if (!finite_all(s.origin, 2) || !rotation_ok(s.rotation)) return fail(LG_E_INVALID, "logic: bad origin or rotation");
+  const Frame g = frame_compose(f, s.origin, s.rotation);
+  const size_t at = recs.size();
+  Rec<double> L;
+  std::memset(&L, 0, sizeof(L));
+  L.kind = REC_LOGIC;
+  L.obj = oi;
+  L.op = s.op;
+  recs.push_back(L);
+  int rc = compile_node(obj, oi, s.child_a, g, recs, verts, leaves);
+  if (rc) return rc;
+  if ((rc = compile_node(obj, oi, s.child_b, g, recs, verts, leaves))) return rc;
+  recs[at].aux = (int32_t)(recs.size() - at);
+  return LG_OK;
+}
+
 static int compile_object(const lg_object& obj, int oi, std::vector<Rec<double>>& recs,
                           std::vector<double>& verts) {
   if (obj.n_shapes < 1 || obj.n_shapes > LG_MAX_SHAPES)
@@ -257,28 +289,16 @@ static int compile_object(const lg_object& obj, int oi, std::vector<Rec<double>>
   const lg_shape& root = obj.shapes[0];
   Frame id = frame_identity();
   if (root.kind == LG_SHAPE_LOGIC) {
-    if (root.op < LG_LOGIC_AND || root.op > LG_LOGIC_AND_NOT)
-      return fail(LG_E_INVALID, "logic: bad op");
-    if (root.child_a <= 0 || root.child_b <= 0 || root.child_a >= obj.n_shapes ||
-        root.child_b >= obj.n_shapes || root.child_a == root.child_b)
-      return fail(LG_E_INVALID, "logic: bad operand index");
-    if (!finite_all(root.origin, 2) || !rotation_ok(root.rotation))
-      return fail(LG_E_INVALID, "logic: bad origin or rotation");
-    const lg_shape& a = obj.shapes[root.child_a];
-    const lg_shape& b = obj.shapes[root.child_b];
-    if (!is_closed_prim(a.kind) || !is_closed_prim(b.kind))
-      return fail(LG_E_UNSUPPORTED,
-                  "logic: operands must be circles, rects, ellipses or convex polygons");
-    Frame f = frame_compose(id, root.origin, root.rotation);
-    Rec<double> L;
-    std::memset(&L, 0, sizeof(L));
-    L.kind = REC_LOGIC;
-    L.obj = oi;
-    L.op = root.op;
-    recs.push_back(L);
-    int rc = compile_prim(obj, oi, a, f, recs, verts);
+    const size_t top = recs.size();
+    std::vector<int> leaves;
+    const int rc = compile_node(obj, oi, 0, id, recs, verts, leaves);
     if (rc) return rc;
-    return compile_prim(obj, oi, b, f, recs, verts);
+    if ((int)leaves.size() > kLogicLeaves) return fail(LG_E_UNSUPPORTED, "logic: more than 4 primitives");
+    Rec<double>& R = recs[top];
+    if (R.aux != 3) R.kind = REC_TREE;
+    for (size_t k = 0; k < leaves.size(); k++) R.p[k] = (double)(leaves[k] - (int)top);
+    R.p[4] = (double)leaves.size();
+    return LG_OK;
   }
   return compile_prim(obj, oi, root, id, recs, verts);
 }
@@ -398,9 +418,9 @@ static std::vector<Box2> object_boxes(const std::vector<Rec<double>>& recs, cons
   for (int o = 0; o < n; o++) {
     const int top = obj_rec[o];
     Box2 b = empty_box();
-    if (recs[top].kind == REC_LOGIC) {
-      b.add(rec_box(recs[top + 1], verts));
-      b.add(rec_box(recs[top + 2], verts));
+    if (recs[top].kind == REC_LOGIC || recs[top].kind == REC_TREE) {
+      for (int k = top + 1; k < top + recs[top].aux; k++)
+        if (recs[k].kind != REC_LOGIC) b.add(rec_box(recs[k], verts));
     } else {
       b = rec_box(recs[top], verts);
     }
@@ -460,6 +480,26 @@ int compile_scene(lg_scene* sc, hipStream_t st) {
     n64.push_back(o.has_material ? o.refractive_index : 1.0);
   }
   if ((int)recs.size() > kMaxRecords) return fail(LG_E_UNSUPPORTED, "scene: too many shape records");
+  // The records of Logic trees of more than two primitives move after every other object's:
+  // the object walk covers the records before the first tree, and the trees are tested in a
+  // loop of their own (records are relocatable: a Logic record refers to its operands by
+  // offset). Objects keep their order within each group.
+  std::vector<int32_t> tree_rec;
+  {
+    std::vector<Rec<double>> ordered;
+    ordered.reserve(recs.size());
+    std::vector<int32_t> at(obj_rec.size());
+    for (int trees = 0; trees < 2; trees++)
+      for (size_t o = 0; o < obj_rec.size(); o++) {
+        if ((recs[obj_rec[o]].kind == REC_TREE) != (trees == 1)) continue;
+        const size_t end = o + 1 < obj_rec.size() ? (size_t)obj_rec[o + 1] : recs.size();
+        at[o] = (int32_t)ordered.size();
+        if (trees) tree_rec.push_back(at[o]);
+        ordered.insert(ordered.end(), recs.begin() + obj_rec[o], recs.begin() + end);
+      }
+    recs.swap(ordered);
+    obj_rec.swap(at);
+  }
   std::vector<Rec<float>> recs32(recs.size());
   for (size_t i = 0; i < recs.size(); i++) {
     recs32[i].kind = recs[i].kind;
@@ -475,6 +515,8 @@ int compile_scene(lg_scene* sc, hipStream_t st) {
   if ((rc = upload(sc->obj_n64, n64.data(), n64.size(), st))) return rc;
   if ((rc = upload(sc->obj_flags, flags.data(), flags.size(), st))) return rc;
   if ((rc = upload(sc->obj_rec, obj_rec.data(), obj_rec.size(), st))) return rc;
+  if ((rc = upload(sc->tree_rec, tree_rec.data(), tree_rec.size(), st))) return rc;
+  sc->n_trees = (int)tree_rec.size();
   if ((rc = upload(sc->verts32, verts32.data(), verts32.size(), st))) return rc;
   if ((rc = upload(sc->verts64, verts.data(), verts.size(), st))) return rc;
   const std::vector<Box2> boxes = object_boxes(recs, verts, obj_rec);
@@ -483,7 +525,14 @@ int compile_scene(lg_scene* sc, hipStream_t st) {
     box32[o] = make_float4(round_down(boxes[o].lo[0]), round_down(boxes[o].lo[1]), round_up(boxes[o].hi[0]),
                            round_up(boxes[o].hi[1]));
   if ((rc = upload(sc->obj_box, box32.data(), box32.size(), st))) return rc;
-  const std::vector<BvhNode> bvh = build_bvh(boxes, obj_rec);
+  std::vector<Box2> walk_boxes;  // the BVH holds the objects of the walk
+  std::vector<int32_t> walk_rec;
+  for (size_t o = 0; o < obj_rec.size(); o++)
+    if (recs[obj_rec[o]].kind != REC_TREE) {
+      walk_boxes.push_back(boxes[o]);
+      walk_rec.push_back(obj_rec[o]);
+    }
+  const std::vector<BvhNode> bvh = build_bvh(walk_boxes, walk_rec);
   if ((rc = upload(sc->bvh_dev, bvh.data(), bvh.size(), st))) return rc;
   sc->n_bvh = (int)bvh.size();
   // lights

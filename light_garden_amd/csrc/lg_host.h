@@ -149,7 +149,9 @@ struct lg_scene {
   std::vector<double> h_verts64;
   int n_recs = 0, n_objs = 0, n_verts = 0;
   lg::DevBuf recs32, recs64, obj_n64, obj_flags, obj_rec, obj_box, verts32, verts64, lights_dev, bvh_dev;
+  lg::DevBuf tree_rec;        // first records of the Logic trees of more than two primitives
   int n_bvh = 0;              // BVH nodes built for the scene (0: none)
+  int n_trees = 0;
   lg::DevBuf q0, q1;          // ray queues (QRay<double>)
   uint64_t q_cap = 0;         // rays per queue
   uint64_t seg_hint = 0;      // high-water segment count

@@ -29,9 +29,11 @@ enum RecKind : int32_t {
   REC_BEZ = LG_SHAPE_CUBIC_BEZIER,      // p: a3 a2 a1 a0 (power basis, x/y pairs)
   REC_CIRCLE = LG_SHAPE_CIRCLE,         // p: cx cy r r^2
   REC_RECT = LG_SHAPE_RECT,             // p: cx cy ux uy vx vy hw hh
-  REC_LOGIC = LG_SHAPE_LOGIC,           // op = lg_logic_op; operands at +1, +2
+  REC_LOGIC = LG_SHAPE_LOGIC,           // op = lg_logic_op; aux = records of the subtree
   REC_POLY = LG_SHAPE_CONVEX_POLYGON,   // p: cx cy ux uy vx vy; aux = first vertex, op = count
-  REC_ELLIPSE = LG_SHAPE_ELLIPSE        // p: cx cy ux uy vx vy a b
+  REC_ELLIPSE = LG_SHAPE_ELLIPSE,       // p: cx cy ux uy vx vy a b
+  REC_TREE = 7                          // the root Logic record of a tree of more than two
+                                        // primitives: as REC_LOGIC, p[0..4] list its leaves
 };
 
 template <typename T>
@@ -85,18 +87,50 @@ __host__ __device__ inline bool contains_prim(const Rec<T>& R, const T* verts, T
   }
 }
 
+// A Logic tree is stored in prefix order: a Logic record (op; aux = records of its subtree,
+// itself included), its operand a's subtree, then b's. A two-primitive object is REC_LOGIC
+// with aux = 3; the root of a larger tree is REC_TREE and also lists its leaves: p[0..3] their
+// record offsets from the root, p[4] their count (at most kLogicLeaves: an object's tree has
+// at most LG_MAX_SHAPES = 7 nodes).
+constexpr int kLogicLeaves = 4;
+
+__host__ __device__ inline bool logic_combine(int op, bool a, bool b) {
+  return op == LG_LOGIC_AND ? (a && b) : op == LG_LOGIC_OR ? (a || b) : (a && !b);
+}
+
+// Containment in a Logic tree of more than two primitives (records [first, first + aux)):
+// the records in reverse prefix order, a primitive pushes its value, a Logic record pops its
+// operands (a on top of b) and pushes their combination.
+template <typename T>
+__host__ __device__ inline bool contains_tree(const Rec<T>* recs, const T* verts, int first, T px, T py) {
+  const Rec<T>& R = recs[first];
+  uint32_t st = 0;
+  int sp = 0;
+  for (int k = first + R.aux - 1; k >= first; k--) {
+    const Rec<T>& Q = recs[k];
+    bool v;
+    if (Q.kind == REC_LOGIC || Q.kind == REC_TREE) {
+      const bool a = (st >> (sp - 1)) & 1u, b = (st >> (sp - 2)) & 1u;
+      sp -= 2;
+      v = logic_combine(Q.op, a, b);
+    } else {
+      v = contains_prim(Q, verts, px, py);
+    }
+    st = (st & ~(1u << sp)) | ((uint32_t)v << sp);
+    sp++;
+  }
+  return st & 1u;
+}
+
 // Object::contains (object.rs:240-259) over the object's records starting at `first`.
 template <typename T>
 __host__ __device__ inline bool contains_object(const Rec<T>* recs, const T* verts, int first,
                                                 T px, T py) {
   const Rec<T>& R = recs[first];
-  if (R.kind == REC_LOGIC) {
-    bool ca = contains_prim(recs[first + 1], verts, px, py);
-    bool cb = contains_prim(recs[first + 2], verts, px, py);
-    if (R.op == LG_LOGIC_AND) return ca && cb;
-    if (R.op == LG_LOGIC_OR) return ca || cb;
-    return ca && !cb;
-  }
+  if (R.kind == REC_LOGIC)
+    return logic_combine(R.op, contains_prim(recs[first + 1], verts, px, py),
+                         contains_prim(recs[first + 2], verts, px, py));
+  if (R.kind == REC_TREE) return contains_tree(recs, verts, first, px, py);
   return contains_prim(R, verts, px, py);
 }
 
@@ -190,8 +224,10 @@ struct SceneDesc {
   const LG_G double* verts64;
   const LG_G DevLight* lights;
   const LG_G BvhNode* bvh;         // NULL: the culling pass walks every record
+  const LG_G int32_t* tree_rec;    // first record of each object that is a Logic tree of more
+                                   // than two primitives (the object walk skips these)
   int32_t n_bvh;                   // nodes
-  int32_t _pad_bvh;
+  int32_t n_trees;
   int32_t n_recs, n_objs, n_verts, n_lights;
   uint64_t n_primary;
   uint32_t max_bounce;

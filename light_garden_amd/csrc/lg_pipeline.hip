@@ -363,6 +363,8 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
       const bool bvh = L.cap_nodes > 0 && scene_uses_bvh(ctx, sc, L.fp64);
       D.bvh = gbl(bvh ? sc->bvh_dev.as<const BvhNode>() : nullptr);
       D.n_bvh = bvh ? sc->n_bvh : 0;
+      D.tree_rec = gbl(sc->tree_rec.as<const int32_t>());
+      D.n_trees = sc->n_trees;
       D.n_recs = sc->n_recs;
       D.n_objs = sc->n_objs;
       D.n_verts = sc->n_verts;

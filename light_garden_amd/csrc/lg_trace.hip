@@ -466,6 +466,28 @@ __device__ __forceinline__ void consider(Nearest<T>& nb, T t, int rec, int code,
   }
 }
 
+// `consider` for a Logic tree of more than two primitives. Such trees' records follow every
+// other object's, so on an exact tie in t the scene order of the objects decides: `first` says
+// that the tree's object comes before the current winner's.
+template <typename T>
+__device__ __forceinline__ void consider_tree(Nearest<T>& nb, T t, int rec, int code, int top, bool first) {
+  if (!(t > tmin_v<T>())) return;
+  if (t < nb.t || (t == nb.t && first)) {
+    if (top != nb.top) {
+      nb.t2 = nb.t;
+      nb.top2 = nb.top;
+    }
+    nb.t = t;
+    nb.rec = rec;
+    nb.top = top;
+    nb.code = code;
+    nb.s = (T)0;
+  } else if (t < nb.t2 && top != nb.top) {
+    nb.t2 = t;
+    nb.top2 = top;
+  }
+}
+
 // Boolean combination of two chords (closed convex operands). Candidate boundary points
 // are offered to `consider` in a-then-b order.
 template <typename T>
@@ -510,8 +532,97 @@ __device__ __forceinline__ void logic_candidates(Nearest<T>& nb, int op, bool hi
   }
 }
 
-// Tests one object (records starting at i) and returns the index of the next object.
 template <typename T>
+struct TreeHit {
+  T t;
+  int rec;  // -1: nothing kept
+  int code;
+};
+
+// A Logic tree of three or four primitives (records [i, i + n), prefix order). A leaf's
+// entry or exit point is kept by the two-operand rule applied at every Logic node above it:
+// And keeps it inside the other operand, Or outside it, AndNot outside b on the a side and
+// inside a on the b side (normal flipped). Insideness at a point's t comes from the other
+// leaves' chords (convex leaves: inside iff t lies strictly within the chord), combined
+// bottom-up. Returns the object's nearest kept point (the first in leaf order, entry before
+// exit, among equal ones): `consider` never makes an object its own runner-up, so that is
+// what offering every kept point would leave. Trees this size are rare, so the chords are
+// recomputed rather than held in registers.
+template <typename T>
+__device__ __forceinline__ TreeHit<T> logic_tree(const RayT<T>& r, const Rec<T>* recs, const Rec<double>* recs64,
+                                                 const T* verts, int i, int n) {
+  const int nl = (int)recs[i].p[4];
+  T best = (T)0;
+  int best_k = -1, best_code = 0;
+  for (int m = 0; m < nl; m++) {
+    const int leaf = i + (int)recs[i].p[m];
+    T tin, tout;
+    int ain, aout;
+    if (!chord_any(r, recs[leaf], recs64 + leaf, verts, tin, tout, ain, aout)) continue;
+    for (int side = 0; side < 2; side++) {
+      const T t = side ? tout : tin;
+      if (!(t > tmin_v<T>()) || (best_k >= 0 && !(t < best))) continue;
+      uint32_t leaf_in = 0;  // the point lies on leaf m's boundary: not inside it
+      for (int q = 0; q < nl; q++) {
+        if (q == m) continue;
+        const int kq = i + (int)recs[i].p[q];
+        T qin, qout;
+        int qa, qb;
+        if (chord_any(r, recs[kq], recs64 + kq, verts, qin, qout, qa, qb) && t > qin && t < qout)
+          leaf_in |= 1u << q;
+      }
+      // insideness of every subtree at t (bit k - i for the subtree whose root record is k)
+      uint32_t node_in = 0, st = 0;
+      int sp = 0, lq = nl - 1;
+      for (int k = i + n - 1; k >= i; k--) {
+        bool v;
+        if (recs[k].kind == REC_LOGIC || recs[k].kind == REC_TREE) {
+          const bool a = (st >> (sp - 1)) & 1u, b = (st >> (sp - 2)) & 1u;
+          sp -= 2;
+          v = logic_combine(recs[k].op, a, b);
+        } else {
+          v = (leaf_in >> lq) & 1u;
+          lq--;
+        }
+        st = (st & ~(1u << sp)) | ((uint32_t)v << sp);
+        sp++;
+        node_in |= (uint32_t)v << (k - i);
+      }
+      // the rules of the Logic nodes from the root down to the leaf
+      bool keep = true, flip = false;
+      int node = i;
+      while (node != leaf) {
+        const int a = node + 1;
+        const int b = a + (recs[a].kind == REC_LOGIC ? recs[a].aux : 1);
+        const bool a_side = leaf < b;
+        const bool other_in = (node_in >> ((a_side ? b : a) - i)) & 1u;
+        const int op = recs[node].op;
+        if (op == LG_LOGIC_AND) {
+          keep = keep && other_in;
+        } else if (op == LG_LOGIC_OR) {
+          keep = keep && !other_in;
+        } else if (a_side) {
+          keep = keep && !other_in;
+        } else {
+          keep = keep && other_in;
+          flip = !flip;
+        }
+        node = a_side ? a : b;
+      }
+      if (keep) {
+        best = t;
+        best_k = leaf;
+        best_code = side | ((side ? aout : ain) << 8) | (flip ? 2 : 0);
+      }
+    }
+  }
+  return TreeHit<T>{best, best_k, best_code};
+}
+
+// Tests one object (records starting at i) and returns the index of the next object. The
+// object walk (kTrees false) never meets a Logic tree of more than two primitives: their
+// records follow the walk's.
+template <typename T, bool kTrees = true>
 __device__ __forceinline__ int eval_object(const RayT<T>& r, const Rec<T>* recs,
                                            const Rec<double>* recs64, const T* verts, int i,
                                            Nearest<T>& nb) {
@@ -539,6 +650,13 @@ __device__ __forceinline__ int eval_object(const RayT<T>& r, const Rec<T>* recs,
     }
     if (hit) consider(nb, t, i, 0, s, i);
     return i + 1;
+  }
+  if constexpr (kTrees) {
+    if (kind == REC_TREE) {
+      const TreeHit<T> h = logic_tree<T>(r, recs, recs64, verts, i, R.aux);
+      if (h.rec >= 0) consider_tree(nb, h.t, h.rec, h.code, i, nb.top < 0 || R.obj < recs[nb.top].obj);
+      return i + R.aux;
+    }
   }
   if (kind == REC_LOGIC) {
     T inA = 0, outA = 0, inB = 0, outB = 0;
@@ -568,11 +686,11 @@ __device__ __forceinline__ int eval_object(const RayT<T>& r, const Rec<T>* recs,
 // padded beyond the FP32 error), so the winner, and any runner-up that the FP64 refinement
 // looks at (within 1e-5 (1 + t)), are those of the full walk. One loop serves both walks so
 // that eval_object is inlined once. Inlined, so that the compiler sees which records are in
-// LDS and which in global memory.
+// LDS and which in global memory. Logic trees of more than two primitives are skipped here.
 template <typename T>
-__device__ __forceinline__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>* recs,
-                                                  const Rec<double>* recs64, const T* verts, int n_recs,
-                                                  const BvhNode* nodes = nullptr, int n_nodes = 0) {
+__device__ __forceinline__ Nearest<T> walk_objects(const RayT<T>& r, const Rec<T>* recs,
+                                                   const Rec<double>* recs64, const T* verts, int n_recs,
+                                                   const BvhNode* nodes, int n_nodes) {
   Nearest<T> nb;
   nearest_init(nb);
   const float ox = (float)r.ox, oy = (float)r.oy, dx = (float)r.dx, dy = (float)r.dy;
@@ -608,8 +726,48 @@ __device__ __forceinline__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>
       top = __builtin_amdgcn_readfirstlane(nodes[leaf].top[k]);
       k++;
     }
-    rec = eval_object(r, recs, recs64, verts, top, nb);
+    rec = eval_object<T, false>(r, recs, recs64, verts, top, nb);
   }
+}
+
+// The objects that are Logic trees of more than two primitives: first records and the
+// per-object boxes (padded world boxes, indexed by the records' object).
+struct TreeList {
+  const int32_t* top;
+  const float4* box;
+  int n;
+};
+
+// Nearest hit over every object: the object walk over the records before the first Logic tree
+// of more than two primitives, then those trees in a wave-uniform loop of their own, where the
+// walk's state is no longer live (such trees are rare; a test for them inside the walk raises
+// its register pressure). `consider` keeps the lexicographic minimum of (t, scene order) and
+// the nearest t of another object, so the order in which objects are offered does not change
+// the result. A tree whose box the ray does not enter within the window of the BVH walk is
+// skipped, for the same reason.
+template <typename T>
+__device__ __forceinline__ Nearest<T> nearest_hit(const RayT<T>& r, const Rec<T>* recs,
+                                                  const Rec<double>* recs64, const T* verts, int n_recs,
+                                                  const TreeList& trees, const BvhNode* nodes = nullptr,
+                                                  int n_nodes = 0) {
+  const int n_walk = trees.n ? __builtin_amdgcn_readfirstlane(trees.top[0]) : n_recs;
+  Nearest<T> nb = walk_objects(r, recs, recs64, verts, n_walk, nodes, n_nodes);
+  if (trees.n == 0) return nb;
+  const T ix = (T)1 / (fabs(r.dx) > (T)1e-30 ? r.dx : (T)__builtin_copysign(1e-30, (double)r.dx));
+  const T iy = (T)1 / (fabs(r.dy) > (T)1e-30 ? r.dy : (T)__builtin_copysign(1e-30, (double)r.dy));
+  for (int k = 0; k < trees.n; k++) {
+    const int top = __builtin_amdgcn_readfirstlane(trees.top[k]);
+    const float4 bx = trees.box[recs[top].obj];
+    const T tx0 = ((T)bx.x - r.ox) * ix, tx1 = ((T)bx.z - r.ox) * ix;
+    const T ty0 = ((T)bx.y - r.oy) * iy, ty1 = ((T)bx.w - r.oy) * iy;
+    const T tn = fmax(fmin(tx0, tx1), fmin(ty0, ty1));
+    const T tf = fmin(fmax(tx0, tx1), fmax(ty0, ty1));
+    const T lim = nb.t + (T)4e-5 * ((T)1 + nb.t);
+    if (!(tn <= tf && tf >= (T)1e-5 && tn <= lim)) continue;
+    const TreeHit<T> h = logic_tree<T>(r, recs, recs64, verts, top, recs[top].aux);
+    if (h.rec >= 0) consider_tree(nb, h.t, h.rec, h.code, top, nb.top < 0 || recs[top].obj < recs[nb.top].obj);
+  }
+  return nb;
 }
 
 // Outward (closed) or left (open) unit normal of the winning surface at p.
@@ -804,14 +962,14 @@ __device__ __forceinline__ double tie_tol(double t) { return 1e-5 * (1.0 + t); }
 // Full FP64 scan for the rare ray whose FP32 winner was not confirmed; out of line to keep
 // the main path's registers and code small.
 __device__ __forceinline__ Nearest<double> rescan_fp64(const RayT<double>& rd, const Rec<double>* recs64,
-                                                    const double* verts64, int n_recs) {
-  return nearest_hit<double>(rd, recs64, recs64, verts64, n_recs);
+                                                    const double* verts64, int n_recs, const TreeList& trees) {
+  return nearest_hit<double>(rd, recs64, recs64, verts64, n_recs, trees);
 }
 
 // FP64 answer for a ray whose FP32 culling pass found `nb`.
 __device__ __forceinline__ void refine_fp64(const RayT<double>& rd, const Nearest<float>& nb,
                                             const Rec<double>* recs64, const double* verts64,
-                                            int n_recs, Nearest<double>& h) {
+                                            int n_recs, const TreeList& trees, Nearest<double>& h) {
   const bool tie = nb.top2 >= 0 && (double)nb.t2 <= (double)nb.t + tie_tol(nb.t);
   if (tie) {
     int a = nb.top < nb.top2 ? nb.top : nb.top2;
@@ -822,7 +980,7 @@ __device__ __forceinline__ void refine_fp64(const RayT<double>& rd, const Neares
     eval_object<double>(rd, recs64, recs64, verts64, nb.top, h);
   }
   const bool unsure = h.top < 0 || (!tie && nb.top2 >= 0 && h.t > (double)nb.t2 - tie_tol(nb.t2));
-  if (unsure) h = rescan_fp64(rd, recs64, verts64, n_recs);
+  if (unsure) h = rescan_fp64(rd, recs64, verts64, n_recs, trees);
 }
 
 __device__ __forceinline__ uint64_t gen_input_count(const SceneDesc& D, uint32_t gen_ix) {
@@ -939,6 +1097,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const Rec<double>* recs64 = gen(D.recs64);
     const double* verts64 = gen(D.verts64);
     const int n_recs = D.n_recs, n_objs = D.n_objs;
+    const TreeList trees{gen(D.tree_rec), gen(D.obj_box), D.n_trees};
     if (sid != loaded) {
       const int4* src = reinterpret_cast<const int4*>(gen(D.recs));
       int4* dst = reinterpret_cast<int4*>(s_rec);
@@ -1000,12 +1159,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         const RayT<double> rd{ox, oy, dx, dy};
         Nearest<double> h;
         if constexpr (sizeof(T) == 8) {
-          h = nearest_hit<double>(rd, s_rec, recs64, s_verts, n_recs);
+          h = nearest_hit<double>(rd, s_rec, recs64, s_verts, n_recs, trees);
         } else {
           const RayT<float> rf{(float)ox, (float)oy, (float)dx, (float)dy};
-          Nearest<float> nb = nearest_hit<float>(rf, s_rec, recs64, s_verts, n_recs, s_nodes, D.n_bvh);
+          Nearest<float> nb = nearest_hit<float>(rf, s_rec, recs64, s_verts, n_recs, trees, s_nodes, D.n_bvh);
           nearest_init(h);
-          if (nb.top >= 0) refine_fp64(rd, nb, recs64, verts64, n_recs, h);
+          if (nb.top >= 0) refine_fp64(rd, nb, recs64, verts64, n_recs, trees, h);
         }
         if (h.rec >= 0) {
           const Rec<double>& R = recs64[h.rec];

@@ -179,6 +179,110 @@ def test_trace_shape_zoo(ctx, orc, fp64):
     assert m["max_err"] < (1e-9 if fp64 else 1e-6)
 
 
+def nested_logic_zoo():
+    """Logic trees of three and four primitives, two and three levels deep, every op at every
+    level, subtrees on the a and on the b side (AndNot's flip composes), Logic nodes with
+    frames of their own."""
+    from light_garden_amd import geo as G
+    rng = np.random.default_rng(47)
+    ops = (abi.LOGIC_AND, abi.LOGIC_OR, abi.LOGIC_AND_NOT)
+
+    def prim(k):
+        c = (rng.uniform(-0.08, 0.08), rng.uniform(-0.08, 0.08))
+        if k == 0:
+            return G.Circle(c, rng.uniform(0.08, 0.16))
+        if k == 1:
+            return G.Rect(c, G.rotation(rng.uniform(0, 3)), rng.uniform(0.1, 0.3), rng.uniform(0.1, 0.3))
+        if k == 2:
+            return G.Ellipse(c, rng.uniform(0.08, 0.2), rng.uniform(0.05, 0.12), G.rotation(rng.uniform(0, 3)))
+        return G.ConvexPolygon.new_convex_hull([(rng.uniform(-0.16, 0.16), rng.uniform(-0.16, 0.16))
+                                                for _ in range(6)])
+
+    def inner(op):
+        frame = (rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05))
+        return G.Logic(op, prim(rng.integers(4)), prim(rng.integers(4)), frame, G.rotation(rng.uniform(0, 3)))
+
+    trees = []
+    for outer in ops:
+        for op in ops:
+            trees.append(G.Logic(outer, inner(op), prim(rng.integers(4))))            # 3 leaves, a side
+            trees.append(G.Logic(outer, prim(rng.integers(4)), inner(op)))            # 3 leaves, b side
+            trees.append(G.Logic(outer, inner(op), inner(ops[(op + 1) % 3])))         # 4 leaves
+    for op in ops:                                                                   # three levels
+        deep = G.Logic(op, G.Logic(ops[(op + 1) % 3], inner(ops[(op + 2) % 3]), prim(0)), prim(1))
+        trees.append(deep)
+        trees.append(G.Logic(ops[(op + 2) % 3], prim(2), G.Logic(op, prim(3), inner(op))))
+    centres = [(x, y) for x in np.linspace(-0.75, 0.75, 6) for y in np.linspace(-0.75, 0.75, 6)]
+    objs = [lg.Object(G.Logic(t.op, t.a, t.b, centres[k], G.rotation(rng.uniform(0, 3))),
+                      lg.Material(rng.uniform(1.3, 1.8)))
+            for k, t in enumerate(trees)]
+    lights = [lg.PointLight((0.013, 0.021), 6000, [0.02, 0.02, 0.02, 0.02]),
+              lg.SpotLight((-0.95, -0.95), 0.6, (1.0, 1.0), 4000, [0.03, 0.01, 0.02, 0.02])]
+    return configs.Scene(objs, lights, max_bounce=8, width=256, height=256, name="nested_logic_zoo")
+
+
+def mixed_logic_scene():
+    """The shape zoo (enough objects for the BVH walk) with nested Logic trees between its
+    objects: tree records follow every other object's, the trees have their own loop."""
+    zoo, trees = shape_zoo(), nested_logic_zoo()
+    objs = []
+    for k, o in enumerate(zoo.objects):
+        objs.append(o)
+        if k % 3 == 1:
+            t = trees.objects[k]
+            objs.append(lg.Object(t.geometry.scaled(0.5, (0.0, 0.0)).translated(0.1, -0.05), t.material_opt))
+    return configs.Scene(objs, zoo.lights, max_bounce=8, width=256, height=256, name="mixed_logic")
+
+
+@pytest.mark.parametrize("fp64", [True, False], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("scene_fn", [nested_logic_zoo, mixed_logic_scene], ids=["trees", "mixed_bvh"])
+def test_trace_nested_logic(ctx, orc, fp64, scene_fn):
+    """Nested Logic trees: every kept hit follows the two-operand rule at each Logic node from
+    the root down to its primitive, as the oracle's recursive filter does; alone and among
+    other objects walked through the BVH."""
+    scene = scene_fn()
+    assert sum(isinstance(o.geometry, lg.Logic) and (isinstance(o.geometry.a, lg.Logic) or isinstance(o.geometry.b, lg.Logic))
+               for o in scene.objects) >= 6
+    pos, col, key = gpu_trace(ctx, scene, fp64=fp64)
+    rpos, rcol, rkey = orc.trace_scene(scene, threads=8)
+    m = orc.match_segments(key, pos, rkey, rpos)
+    print(m)
+    assert m["n_b"] > 20000
+    assert m["matched"] >= 0.999 * m["n_b"]
+    assert m["diverged"] <= 0.001 * m["n_b"]
+    assert m["max_err"] < (1e-9 if fp64 else 1e-6)
+
+
+@pytest.mark.parametrize("tree_first", [True, False], ids=["tree_first", "rect_first"])
+def test_tree_exact_tie_follows_scene_order(ctx, orc, tree_first):
+    """A tree's rect and a larger plain rect share their bottom edge exactly (dyadic
+    coordinates): rays from a point light below meet both at the same t, and the object that
+    comes first in the scene wins, as in the oracle's walk. The two have different refractive
+    indices, so the winner shows in the bend of the refracted rays. Two generations only: a
+    strict inside test at a point of an exactly shared boundary would turn on the last bit of
+    the end point, which FP64 arithmetic in another order may round the other way."""
+    from light_garden_amd import geo as G
+    big = G.Rect((0.0, 0.125), G.IDENTITY, 1.0, 0.75)    # bottom edge y = -0.25, x in [-0.5, 0.5]
+    small = G.Rect((0.0, 0.0), G.IDENTITY, 0.5, 0.5)     # bottom edge y = -0.25, x in [-0.25, 0.25]
+    far = lambda x: G.Circle((x, 0.9), 0.02)
+    tree = lg.Object(G.Logic(abi.LOGIC_OR, G.Logic(abi.LOGIC_OR, small, far(-0.9)), far(0.9)), lg.Material(1.7))
+    rect = lg.Object(big, lg.Material(1.2))
+    objs = [tree, rect] if tree_first else [rect, tree]
+    lights = [lg.PointLight((0.03125, -0.875), 4000, [0.05, 0.05, 0.05, 0.05])]
+    scene = configs.Scene(objs, lights, max_bounce=2, width=128, height=128, name="tree_tie")
+    rpos, rcol, rkey = orc.trace_scene(scene)
+    assert len(rkey) > 4500
+    for fp64 in (True, False):
+        pos, col, key = gpu_trace(ctx, scene, fp64=fp64)
+        m = orc.match_segments(key, pos, rkey, rpos)
+        print(fp64, m)
+        if fp64:
+            assert m["n_a"] == m["n_b"] and m["matched"] == m["n_b"] and m["diverged"] == 0
+            assert m["max_err"] < 1e-9
+        else:
+            assert m["matched"] >= 0.999 * m["n_b"] and m["diverged"] <= 0.001 * m["n_b"]
+
+
 def test_python_trace_all_matches_c_entry(ctx, orc):
     scene = configs.plain_small(num_rays=700)
     tr = make_tracer(ctx, scene, fp64=True)
