@@ -367,6 +367,12 @@ int lg_ctx_get_tuning(const lg_ctx* ctx, lg_tuning* out);
 int lg_ctx_set_tuning(lg_ctx* ctx, const lg_tuning* tuning);
 /* Replay frames from a hipGraph while the batch keeps its shape (default on). */
 int lg_ctx_set_graphs(lg_ctx* ctx, int enable);
+/* Tile accumulation with two colour channels per 64-bit LDS add (default on). It applies to
+ * each scene of a frame whose every deposit is non-negative (colours >= 0, no ReverseSubtract,
+ * no blend factor below 0 on them); other scenes take one channel per add, so a scene's image
+ * does not depend on the scenes batched with it. No reference counterpart (the reference
+ * blends in wgpu). Off: every scene takes one channel per add. */
+int lg_ctx_set_packed_accumulation(lg_ctx* ctx, int enable);
 
 /* ---- serving (no reference counterpart: the reference renders one scene per event-loop
  * frame, LightGarden::draw mod.rs:680-696; this is the batch/serve seam lg_server_*) ----

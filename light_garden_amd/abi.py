@@ -200,6 +200,7 @@ FUNCTIONS = {
     "lg_batch_render": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(LgParams),
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "lg_ctx_set_graphs": (C.c_int, [_P, C.c_int]),
+    "lg_ctx_set_packed_accumulation": (C.c_int, [_P, C.c_int]),
     "lg_ctx_get_tuning": (C.c_int, [_P, C.POINTER(LgTuning)]),
     "lg_ctx_set_tuning": (C.c_int, [_P, C.POINTER(LgTuning)]),
     "lg_server_create": (C.c_int, [_P, C.POINTER(LgServerConfig), _PP]),

@@ -483,6 +483,35 @@ __device__ __forceinline__ int scene_fixed_bits(const SceneDesc& D, int c) {
   return G < -250 ? -250 : (G > 100 ? 100 : G);
 }
 
+// Packed deposits: two channels per 64-bit LDS word, (R, G) and (B, A) as 32-bit fields, for
+// scenes whose every deposit is non-negative (SCENE_PACKED). A sample's value of channel c is
+// cq = round(v * 2^G) <= lim <= 2^23 (so (float)cq is exact) and its two deposits add up to cq;
+// a pair deposits at most once into a pixel, so the n pairs of one item keep a field below
+// n * lim <= 2^32 - 1 and no carry crosses into the other field. Every item of a tile takes G
+// from n = min(pairs of the tile, kPackedPiece) (no item holds more), so the items' sums add up
+// in the merge slot and the launch tuning changes no bit.
+constexpr double kPackLevels = 8388608.0;  // 2^23
+__device__ __forceinline__ int packed_fixed_bits(const SceneDesc& D, int c, uint32_t n) {
+  const double bound = (double)D.color_max * (double)D.factor_max[c];
+  if (!(bound > 0.0) || !(bound < 1e300)) return 0;
+  double lim = floor(4294967295.0 / (double)(n > 1u ? n : 1u));
+  lim = lim < kPackLevels ? lim : kPackLevels;
+  // v <= bound, rounded once in FP32: v * 2^G + 1/2 stays below lim + 1
+  const int G = (int)floor(log2((lim - 0.5) / bound));
+  return G < -250 ? -250 : (G > 100 ? 100 : G);
+}
+
+// The pair count that bounds a packed tile's fields (see packed_fixed_bits).
+__device__ __forceinline__ uint32_t packed_pairs(const uint32_t* tile_start, uint32_t g) {
+  const uint32_t nt = tile_start[g + 1] - tile_start[g];
+  return nt < kPackedPiece ? nt : kPackedPiece;
+}
+
+// One packed channel value of a sample: round(max(v, 0) * 2^G) (a NaN deposits nothing).
+__device__ __forceinline__ uint32_t pack_q(float v, float scale) {
+  return (uint32_t)__builtin_fmaf(__builtin_fmaxf(v, 0.0f), scale, 0.5f);
+}
+
 // The value one sample of colour channel c adds per unit weight: +-c * factor (BlendCode).
 __device__ __forceinline__ float blend_value(uint32_t code, float c, float a, float k) {
   float f;
@@ -526,14 +555,14 @@ __device__ __forceinline__ ushort4 resolve_px(const float4 a) {
   return o;
 }
 
-// In-place transpose of the four 64x64 planes, (r, c) <-> (c, r). Job j of 128 is plane j / 32
-// and diagonal pair d = j % 32 + 1: lanes 0 .. 63 - d swap the elements of diagonal c - r = d,
+// In-place transpose of the first `planes` 64x64 planes, (r, c) <-> (c, r). Job j of 32 * planes is
+// plane j / 32 and diagonal pair d = j % 32 + 1: lanes 0 .. 63 - d swap the elements of diagonal c - r = d,
 // lanes 64 - d .. 63 those of diagonal 64 - d (d < 32; diagonal 32 takes lanes 0 .. 31 alone).
 // Every element off the main diagonal is swapped exactly once, and the lanes' addresses step
 // by 65 elements, which keeps the swaps free of bank conflicts.
 template <int WAVES>
-__device__ __forceinline__ void transpose_tile(unsigned long long* t, int wave, int lane) {
-  for (int j = wave; j < 4 * 32; j += WAVES) {
+__device__ __forceinline__ void transpose_tile(unsigned long long* t, int planes, int wave, int lane) {
+  for (int j = wave; j < planes * 32; j += WAVES) {
     const int d = (j & 31) + 1;
     unsigned long long* pl = t + (j >> 5) * kPlane;
     const bool first_diag = lane < kTile - d;
@@ -564,10 +593,14 @@ __device__ __forceinline__ void transpose_tile(unsigned long long* t, int wave, 
 // (k - a) * slope in FP32 relative to the tile's minor origin, m_first rounded once from FP64.
 // Two tiles sharing a sample (its two deposits on either side of a tile edge) differ only by
 // FP32 rounding (< 1e-5 px).
+//
+// A scene with SCENE_PACKED (all its deposits non-negative) keeps two channels per 64-bit word
+// in the first two planes (packed_fixed_bits); other scenes one channel per word in four.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __restrict__ descs,
                                                           const uint32_t* __restrict__ pairs,
                                                           const WorkItem* __restrict__ items,
+                                                          const uint32_t* __restrict__ tile_start,
                                                           const uint32_t* __restrict__ tile_split,
                                                           unsigned long long* split_fx, BatchCtl* ctl) {
   // one guard row, then the four planes
@@ -597,9 +630,16 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
     const float4* __restrict__ col0 = gen(D.seg_col);
     const float4* __restrict__ col1 = gen(D.seg_col1);
     float4* accum = gen(D.accum);
+    const bool two_ch = (D.flags & SCENE_PACKED) != 0;  // two channels per word; uniform
     int G[4];
+    if (two_ch) {
+      const uint32_t n = packed_pairs(tile_start, D.tile_base + W.tile);
 #pragma unroll
-    for (int c = 0; c < 4; c++) G[c] = scene_fixed_bits(D, c);
+      for (int c = 0; c < 4; c++) G[c] = packed_fixed_bits(D, c, n);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) G[c] = scene_fixed_bits(D, c);
+    }
     const float cs0 = __builtin_ldexpf(1.0f, G[0]), cs1 = __builtin_ldexpf(1.0f, G[1]);
     const float cs2 = __builtin_ldexpf(1.0f, G[2]), cs3 = __builtin_ldexpf(1.0f, G[3]);
     const uint32_t bc = __builtin_amdgcn_readfirstlane(D.blend_code);
@@ -675,10 +715,17 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
               const int vc = (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f || dc.w != 0.f) ? 1 : 0;
               s_first = (float)(u0 * g.inv_dM);
               ds = (float)g.inv_dM;
-              cq0 = fx(blend_value(bc0, ca.x, ca.w, bk0), cs0);
-              cq1 = fx(blend_value(bc1, ca.y, ca.w, bk1), cs1);
-              cq2 = fx(blend_value(bc2, ca.z, ca.w, bk2), cs2);
-              cq3 = fx(blend_value(bc3, ca.w, ca.w, bk3), cs3);
+              if (two_ch) {
+                cq0 = (int)pack_q(blend_value(bc0, ca.x, ca.w, bk0), cs0);
+                cq1 = (int)pack_q(blend_value(bc1, ca.y, ca.w, bk1), cs1);
+                cq2 = (int)pack_q(blend_value(bc2, ca.z, ca.w, bk2), cs2);
+                cq3 = (int)pack_q(blend_value(bc3, ca.w, ca.w, bk3), cs3);
+              } else {
+                cq0 = fx(blend_value(bc0, ca.x, ca.w, bk0), cs0);
+                cq1 = fx(blend_value(bc1, ca.y, ca.w, bk1), cs1);
+                cq2 = fx(blend_value(bc2, ca.z, ca.w, bk2), cs2);
+                cq3 = fx(blend_value(bc3, ca.w, ca.w, bk3), cs3);
+              }
               const int neg = (cq0 | cq1 | cq2 | cq3) < 0 ? 1 : 0;
               packed = (int)(b - a + 1) | ((int)(a - maj0) << 8) | (g.xmajor << 16) | (vc << 17) | (neg << 18);
             }
@@ -710,9 +757,46 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
           // position and deposit nothing.
           const float m = __builtin_fmaf(lane_f, jsl, jm);
           const int jl = (int)__builtin_floorf(m);
+          const int e0 = jl * kTile + lane + jq0;
+          if (two_ch) {
+            // pixel jl takes round((1 - f) cq) of each channel, pixel jl + 1 the rest of cq
+            unsigned long long* const t = tile + e0;
+            const bool lo_in = lane < jn && (unsigned)jl < (unsigned)kTile;
+            const bool hi_in = lane < jn && (unsigned)(jl + 1) < (unsigned)kTile;
+            const float wlf = 1.0f - __builtin_amdgcn_fractf(m);
+            uint32_t q0, q1, q2, q3;
+            if (!jvc) {
+              q0 = (uint32_t)bcast_i(cq0, j);
+              q1 = (uint32_t)bcast_i(cq1, j);
+              q2 = (uint32_t)bcast_i(cq2, j);
+              q3 = (uint32_t)bcast_i(cq3, j);
+            } else {
+              const float jds = bcast_f(ds, j), jsa = bcast_f(s_first, j);
+              const float a0 = bcast_f(ca.x, j), a1 = bcast_f(ca.y, j), a2 = bcast_f(ca.z, j), a3 = bcast_f(ca.w, j);
+              const float d0 = bcast_f(dc.x, j), d1 = bcast_f(dc.y, j), d2 = bcast_f(dc.z, j), d3 = bcast_f(dc.w, j);
+              const float sp = __builtin_fmaf(lane_f, jds, jsa);
+              const float al = __builtin_fmaf(sp, d3, a3);
+              q0 = pack_q(blend_value(bc0, __builtin_fmaf(sp, d0, a0), al, bk0), cs0);
+              q1 = pack_q(blend_value(bc1, __builtin_fmaf(sp, d1, a1), al, bk1), cs1);
+              q2 = pack_q(blend_value(bc2, __builtin_fmaf(sp, d2, a2), al, bk2), cs2);
+              q3 = pack_q(blend_value(bc3, al, al, bk3), cs3);
+            }
+            const uint32_t l0 = (uint32_t)__builtin_fmaf(wlf, (float)q0, 0.5f);
+            const uint32_t l1 = (uint32_t)__builtin_fmaf(wlf, (float)q1, 0.5f);
+            const uint32_t l2 = (uint32_t)__builtin_fmaf(wlf, (float)q2, 0.5f);
+            const uint32_t l3 = (uint32_t)__builtin_fmaf(wlf, (float)q3, 0.5f);
+            if (lo_in) {
+              atomicAdd(t, (unsigned long long)l0 | ((unsigned long long)l1 << 32));
+              atomicAdd(t + kPlane, (unsigned long long)l2 | ((unsigned long long)l3 << 32));
+            }
+            if (hi_in) {
+              atomicAdd(t + kTile, (unsigned long long)(q0 - l0) | ((unsigned long long)(q1 - l1) << 32));
+              atomicAdd(t + kTile + kPlane, (unsigned long long)(q2 - l2) | ((unsigned long long)(q3 - l3) << 32));
+            }
+            continue;
+          }
           const int wq = (int)__builtin_fmaf(__builtin_amdgcn_fractf(m), (float)kWOne, 0.5f);
           const int wl = kWOne - wq;
-          const int e0 = jl * kTile + lane + jq0;
           // plane 2's entry as an opaque register: the deposits of planes 2 and 3 then address
           // it with the instruction's offset field instead of each adding 64 KB and more
           int e2 = e0 + 2 * kPlane;
@@ -797,7 +881,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
       }
       __syncthreads();
       if (phase == 0) {
-        transpose_tile<kWaves>(tile, wave, lane);
+        transpose_tile<kWaves>(tile, two_ch ? 2 : 4, wave, lane);
         __syncthreads();
       }
     }
@@ -807,14 +891,16 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
       for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-          const unsigned long long val = tile[c * kPlane + k];
+          const unsigned long long val =
+              two_ch ? (tile[(c >> 1) * kPlane + k] >> (32 * (c & 1))) & 0xffffffffull : tile[c * kPlane + k];
           if (val) atomicAdd(&fx[c * kTile * kTile + k], val);
         }
       }
     } else {
       // the only item of its tile: fixed point -> FP32 accumulator
-      const double un0 = __builtin_ldexp(1.0, -(G[0] + kWBits)), un1 = __builtin_ldexp(1.0, -(G[1] + kWBits));
-      const double un2 = __builtin_ldexp(1.0, -(G[2] + kWBits)), un3 = __builtin_ldexp(1.0, -(G[3] + kWBits));
+      const int wb = two_ch ? 0 : kWBits;
+      const double un0 = __builtin_ldexp(1.0, -(G[0] + wb)), un1 = __builtin_ldexp(1.0, -(G[1] + wb));
+      const double un2 = __builtin_ldexp(1.0, -(G[2] + wb)), un3 = __builtin_ldexp(1.0, -(G[3] + wb));
       for (int k = threadIdx.x; k < kTile * kTile; k += BLOCK) {
         int r = k / kTile, cc = k % kTile;
         int gx = x0 + cc, gy = y0 + r;
@@ -822,10 +908,18 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
         const int idx = k;  // row-major
         const size_t px = (size_t)gy * v.width + gx;
         float4 o = whole ? make_float4(0.f, 0.f, 0.f, 0.f) : accum[px];
-        o.x += (float)((double)(long long)tile[idx] * un0);
-        o.y += (float)((double)(long long)tile[kPlane + idx] * un1);
-        o.z += (float)((double)(long long)tile[2 * kPlane + idx] * un2);
-        o.w += (float)((double)(long long)tile[3 * kPlane + idx] * un3);
+        if (two_ch) {
+          const unsigned long long w0 = tile[idx], w1 = tile[kPlane + idx];
+          o.x += (float)((double)(uint32_t)w0 * un0);
+          o.y += (float)((double)(uint32_t)(w0 >> 32) * un1);
+          o.z += (float)((double)(uint32_t)w1 * un2);
+          o.w += (float)((double)(uint32_t)(w1 >> 32) * un3);
+        } else {
+          o.x += (float)((double)(long long)tile[idx] * un0);
+          o.y += (float)((double)(long long)tile[kPlane + idx] * un1);
+          o.z += (float)((double)(long long)tile[2 * kPlane + idx] * un2);
+          o.w += (float)((double)(long long)tile[3 * kPlane + idx] * un3);
+        }
         accum[px] = o;
         if (whole) gen(D.out16)[px] = resolve_px(o);
       }
@@ -839,6 +933,7 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
 // the unsplit flush) and leaves the slots zeroed for the next frame. Persistent over slots.
 __global__ __launch_bounds__(256) void tile_merge_kernel(const SceneDesc* __restrict__ descs,
                                                         const uint32_t* __restrict__ tile_scene,
+                                                        const uint32_t* __restrict__ tile_start,
                                                         const uint32_t* __restrict__ split_tiles,
                                                         const BatchCtl* ctl, unsigned long long* split_fx,
                                                         uint32_t split_cap) {
@@ -852,8 +947,14 @@ __global__ __launch_bounds__(256) void tile_merge_kernel(const SceneDesc* __rest
     const uint32_t t = g - (uint32_t)D.tile_base;
     const int x0 = (int)(t % (uint32_t)v.tiles_x) * kTile, y0 = (int)(t / (uint32_t)v.tiles_x) * kTile;
     double un[4];
+    if (D.flags & SCENE_PACKED) {
+      const uint32_t np = packed_pairs(tile_start, g);
 #pragma unroll
-    for (int c = 0; c < 4; c++) un[c] = __builtin_ldexp(1.0, -(scene_fixed_bits(D, c) + kWBits));
+      for (int c = 0; c < 4; c++) un[c] = __builtin_ldexp(1.0, -packed_fixed_bits(D, c, np));
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) un[c] = __builtin_ldexp(1.0, -(scene_fixed_bits(D, c) + kWBits));
+    }
     const bool whole = (D.flags & SCENE_FRESH_IMAGE) != 0;  // the frame clears and resolves here
     unsigned long long* fx = split_fx + (size_t)sl * kSlotU64;
     for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
@@ -989,17 +1090,17 @@ hipError_t launch_work_items(const SceneDesc* descs, const uint32_t* tile_scene,
 }
 
 hipError_t launch_tile_accum(const SceneDesc* descs, const uint32_t* pairs, const WorkItem* items,
-                             const uint32_t* tile_split, unsigned long long* split_fx, BatchCtl* ctl, int grid,
-                             hipStream_t st) {
+                             const uint32_t* tile_start, const uint32_t* tile_split, unsigned long long* split_fx,
+                             BatchCtl* ctl, int grid, hipStream_t st) {
   hipLaunchKernelGGL((tile_accum_kernel<kTileBlock>), dim3(grid), dim3(kTileBlock), 0, st, descs, pairs, items,
-                     tile_split, split_fx, ctl);
+                     tile_start, tile_split, split_fx, ctl);
   return hipGetLastError();
 }
 
-hipError_t launch_tile_merge(const SceneDesc* descs, const uint32_t* tile_scene, const uint32_t* split_tiles,
-                             const BatchCtl* ctl, unsigned long long* split_fx, uint32_t split_cap, int grid,
-                             hipStream_t st) {
-  hipLaunchKernelGGL(tile_merge_kernel, dim3(grid), dim3(256), 0, st, descs, tile_scene, split_tiles, ctl,
+hipError_t launch_tile_merge(const SceneDesc* descs, const uint32_t* tile_scene, const uint32_t* tile_start,
+                             const uint32_t* split_tiles, const BatchCtl* ctl, unsigned long long* split_fx,
+                             uint32_t split_cap, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(tile_merge_kernel, dim3(grid), dim3(256), 0, st, descs, tile_scene, tile_start, split_tiles, ctl,
                      split_fx, split_cap);
   return hipGetLastError();
 }

@@ -581,9 +581,14 @@ int compile_scene(lg_scene* sc, hipStream_t st) {
   if ((rc = upload(sc->lights_dev, dl.data(), dl.size(), st))) return rc;
   sc->n_primary = base;
   float lmax = 0.0f;
+  bool nonneg = true;
   for (const lg_light& L : sc->lights)
-    for (int c = 0; c < 4; c++) lmax = std::max(lmax, std::fabs(L.color[c]));
+    for (int c = 0; c < 4; c++) {
+      lmax = std::max(lmax, std::fabs(L.color[c]));
+      nonneg = nonneg && L.color[c] >= 0.0f;
+    }
   sc->light_max = lmax;
+  sc->light_nonneg = nonneg;
   sc->h_recs64 = recs;
   sc->h_obj_rec = obj_rec;
   sc->h_verts64 = verts;
@@ -859,6 +864,8 @@ int lg_segments_upload(lg_segments* s, const float* pos, const float* color0, co
   hipStream_t st = s->ctx->stream;
   int rc = seg_ensure(s, std::max<uint64_t>(n, 1), false);
   if (rc) return rc;
+  s->max_abs = 0.0f;
+  s->nonneg = true;
   if (n) {
     LG_HIP(hipMemcpyAsync(s->pos.p, pos, n * 16, hipMemcpyHostToDevice, st));
     LG_HIP(hipMemcpyAsync(s->col0.p, color0, n * 16, hipMemcpyHostToDevice, st));
@@ -867,10 +874,18 @@ int lg_segments_upload(lg_segments* s, const float* pos, const float* color0, co
     std::vector<uint64_t> keys(n);
     std::iota(keys.begin(), keys.end(), 0ull);
     float mx = 0.0f;
-    for (uint64_t i = 0; i < 4 * n; i++) mx = std::max(mx, std::fabs(color0[i]));
+    bool nonneg = true;
+    for (uint64_t i = 0; i < 4 * n; i++) {
+      mx = std::max(mx, std::fabs(color0[i]));
+      nonneg = nonneg && color0[i] >= 0.0f;
+    }
     if (color1)
-      for (uint64_t i = 0; i < 4 * n; i++) mx = std::max(mx, std::fabs(color1[i]));
+      for (uint64_t i = 0; i < 4 * n; i++) {
+        mx = std::max(mx, std::fabs(color1[i]));
+        nonneg = nonneg && color1[i] >= 0.0f;
+      }
     s->max_abs = mx;
+    s->nonneg = nonneg;
     LG_HIP(hipMemcpyAsync(s->key.p, keys.data(), n * 8, hipMemcpyHostToDevice, st));
     LG_HIP(hipStreamSynchronize(st));
   }
@@ -1094,6 +1109,12 @@ int lg_batch_render(lg_ctx* ctx, uint32_t n, lg_scene* const* scenes, const lg_p
 int lg_ctx_set_graphs(lg_ctx* ctx, int enable) {
   if (!ctx) return fail(LG_E_INVALID, "ctx is NULL");
   ctx->use_graphs = enable != 0;
+  return LG_OK;
+}
+
+int lg_ctx_set_packed_accumulation(lg_ctx* ctx, int enable) {
+  if (!ctx) return fail(LG_E_INVALID, "ctx is NULL");
+  ctx->packed_accum = enable != 0;
   return LG_OK;
 }
 

@@ -28,6 +28,27 @@ void blend_factor_bounds(uint32_t code, float m, const float k[4], float out[4])
   }
 }
 
+bool blend_nonneg(uint32_t code, float m, const float k[4]) {
+  for (int c = 0; c < 4; c++) {
+    const uint32_t cc = (code >> (4 * c)) & 15u;
+    if (cc & BC_NEGATE) return false;
+    switch (cc & 7u) {
+      case BC_ONE_MINUS_SRC:
+      case BC_ONE_MINUS_SRC_ALPHA:
+        if (!(m <= 1.0f)) return false;
+        break;
+      case BC_CONST:
+        if (!(k[c] >= 0.0f)) return false;
+        break;
+      case BC_ONE_MINUS_CONST:
+        if (!(k[c] <= 1.0f)) return false;
+        break;
+      default: break;
+    }
+  }
+  return true;
+}
+
 }  // namespace lg
 
 namespace {

@@ -99,11 +99,11 @@ hipError_t launch_work_items(const SceneDesc* descs, const uint32_t* tile_scene,
                              uint32_t split_cap, uint64_t pairs_cap, uint32_t min_piece, uint32_t max_piece,
                              uint32_t grid, bool whole_images, hipStream_t st);
 hipError_t launch_tile_accum(const SceneDesc* descs, const uint32_t* pairs, const WorkItem* items,
-                             const uint32_t* tile_split, unsigned long long* split_fx, BatchCtl* ctl, int grid,
-                             hipStream_t st);
-hipError_t launch_tile_merge(const SceneDesc* descs, const uint32_t* tile_scene, const uint32_t* split_tiles,
-                             const BatchCtl* ctl, unsigned long long* split_fx, uint32_t split_cap, int grid,
-                             hipStream_t st);
+                             const uint32_t* tile_start, const uint32_t* tile_split, unsigned long long* split_fx,
+                             BatchCtl* ctl, int grid, hipStream_t st);
+hipError_t launch_tile_merge(const SceneDesc* descs, const uint32_t* tile_scene, const uint32_t* tile_start,
+                             const uint32_t* split_tiles, const BatchCtl* ctl, unsigned long long* split_fx,
+                             uint32_t split_cap, int grid, hipStream_t st);
 uint64_t merge_slot_u64();
 hipError_t launch_pixels(int mode, const SceneDesc* descs, const uint64_t* px_pre, int n_scenes, uint64_t n_px,
                          hipStream_t st);
@@ -118,6 +118,7 @@ struct lg_ctx {
   bool closing = false;  // lg_ctx_destroy was called while children were alive
   bool profiling = false;
   bool use_graphs = true;
+  bool packed_accum = true;   // two channels per LDS word for scenes whose deposits are all >= 0
   lg_tuning tune{256u, 4096u, 8192u, 4u, 16u};
   std::vector<lg::StageRec> pending;  // events of the current call
   std::vector<hipEvent_t> event_pool;
@@ -157,6 +158,7 @@ struct lg_scene {
   uint64_t seg_hint = 0;      // high-water segment count
   uint64_t n_primary = 0;
   float light_max = 0.0f;     // largest |light colour channel|
+  bool light_nonneg = true;   // every light colour channel is >= 0 (so is every traced segment's)
 };
 
 struct lg_segments {
@@ -164,6 +166,7 @@ struct lg_segments {
   uint32_t flags = 0;
   uint64_t count = 0, cap = 0;
   float max_abs = 0.0f;       // largest |colour channel| of the stored segments (host-side bound)
+  bool nonneg = true;         // every colour channel of the stored segments is >= 0
   lg::DevBuf pos, col0, col1, key, pos64;
   bool has_pos64 = false;
 };
@@ -204,6 +207,8 @@ void begin_call(lg_ctx* ctx);
 void end_call(lg_ctx* ctx);
 // per channel, max |blend factor| of a BlendCode word over colours with |channel| <= m
 void blend_factor_bounds(uint32_t code, float m, const float k[4], float out[4]);
+// Whether every value colour * factor is >= 0 for colour channels in [0, m].
+bool blend_nonneg(uint32_t code, float m, const float k[4]);
 
 // Runs `stages` for every entry as one batched frame on ctx's stream and waits for it.
 int pipeline_run(lg_ctx* ctx, const std::vector<Entry>& entries, uint32_t stages);

@@ -193,8 +193,9 @@ struct AccumView {
 
 enum SceneFlags : uint32_t {
   SCENE_FIXED_COUNT = 1u,     // accumulate `seg_fixed` uploaded segments (no trace)
-  SCENE_FRESH_IMAGE = 2u      // the frame clears, accumulates and resolves the image: the tile
+  SCENE_FRESH_IMAGE = 2u,     // the frame clears, accumulates and resolves the image: the tile
                               // flushes write the accumulator and the RGBA16F target directly
+  SCENE_PACKED = 4u           // every deposit is non-negative: two channels per LDS word
 };
 
 // Device pointers kept in descriptors carry the global address space in their type: kernels
@@ -307,6 +308,10 @@ struct WorkItem {
   uint32_t begin, end;        // range in the pair list
 };
 constexpr uint32_t kSplitBit = 0x80000000u;
+// Pairs per tile work item at most: the fixed point of a tile that accumulates two channels
+// per LDS word follows min(pairs of the tile, kPackedPiece), whatever the launch tuning, so
+// results do not depend on it.
+constexpr uint32_t kPackedPiece = 32768u;
 
 // Work distribution of the persistent kernels: workgroup b first takes item b; workgroups
 // that finish it take further items from `cursor`, which counts up from gridDim.x. A plain

@@ -159,8 +159,8 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const
                                P.cbase.as<uint32_t>(), P.offsets.as<uint32_t>(), d_ctl,
                                P.tile_start.as<uint32_t>(), P.pieces.as<uint32_t>(), P.items.as<WorkItem>(),
                                P.items_cap, P.tile_split.as<uint32_t>(), P.split_tiles.as<uint32_t>(), P.split_cap,
-                               P.pairs_cap, ctx->tune.min_piece, ctx->tune.max_piece, (uint32_t)ctx->n_cu, whole,
-                               st));
+                               P.pairs_cap, std::min(ctx->tune.min_piece, kPackedPiece),
+                               std::min(ctx->tune.max_piece, kPackedPiece), (uint32_t)ctx->n_cu, whole, st));
     }
     {
       Scope s(ctx, profile, "bin_fill", 0);
@@ -169,13 +169,15 @@ static int enqueue(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, const
     }
     {
       Scope s(ctx, profile, "tile_accum", 0);
-      LG_HIP(launch_tile_accum(d_descs, P.pairs.as<uint32_t>(), P.items.as<WorkItem>(), P.tile_split.as<uint32_t>(),
-                               P.split_fx.as<unsigned long long>(), d_ctl, ctx->n_cu, st));
+      LG_HIP(launch_tile_accum(d_descs, P.pairs.as<uint32_t>(), P.items.as<WorkItem>(), P.tile_start.as<uint32_t>(),
+                               P.tile_split.as<uint32_t>(), P.split_fx.as<unsigned long long>(), d_ctl, ctx->n_cu,
+                               st));
     }
     {
       Scope s(ctx, profile, "tile_merge", 0);
-      LG_HIP(launch_tile_merge(d_descs, P.tile_scene.as<uint32_t>(), P.split_tiles.as<uint32_t>(), d_ctl,
-                               P.split_fx.as<unsigned long long>(), P.split_cap, ctx->n_cu, st));
+      LG_HIP(launch_tile_merge(d_descs, P.tile_scene.as<uint32_t>(), P.tile_start.as<uint32_t>(),
+                               P.split_tiles.as<uint32_t>(), d_ctl, P.split_fx.as<unsigned long long>(), P.split_cap,
+                               ctx->n_cu, st));
     }
   }
   if ((stages & ST_RESOLVE) && !whole) {
@@ -349,6 +351,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
     D.seg_count = gbl(reinterpret_cast<unsigned long long*>(P.counters.as<char>() + i * kCounterStride));
     D.gen_count = gbl(reinterpret_cast<uint32_t*>(P.counters.as<char>() + i * kCounterStride + kGenCountOffset));
     float max_abs = 0.0f;
+    bool nonneg = true;
     if (trace) {
       lg_scene* sc = e.scene;
       D.recs = gbl(L.fp64 ? sc->recs64.p : sc->recs32.p);
@@ -377,6 +380,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
       D.q1 = gbl(sc->q1.as<QRay<double>>());
       D.q_cap = sc->q_cap;
       max_abs = sc->light_max;
+      nonneg = sc->light_nonneg;
     }
     if (e.segs) {
       D.seg_pos = gbl(e.segs->pos.as<float4>());
@@ -389,6 +393,7 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
         D.flags |= SCENE_FIXED_COUNT;
         D.seg_fixed = e.segs->count;
         max_abs = e.segs->max_abs;
+        nonneg = e.segs->nonneg;
       }
     }
     if (e.img) {
@@ -405,6 +410,9 @@ static int batch_launch(lg_ctx* ctx, Pipeline& P, hipStream_t st, bool profile, 
     D.blend_code = e.img ? e.img->blend_code : kBlendDefault;
     for (int c = 0; c < 4; c++) D.blend_k[c] = e.img ? e.img->blend.constant[c] : 0.0f;
     blend_factor_bounds(D.blend_code, max_abs, D.blend_k, D.factor_max);
+    // two channels per LDS word when every deposit of the scene is non-negative
+    if (ctx->packed_accum && nonneg && std::isfinite(max_abs) && blend_nonneg(D.blend_code, max_abs, D.blend_k))
+      D.flags |= SCENE_PACKED;
     D.tile_base = L.tile_base[i];
     if (accum) {
       int t = D.view.tiles_x * D.view.tiles_y;
@@ -517,6 +525,7 @@ static int batch_finish(lg_ctx* ctx, Pipeline& P, bool profile, Batch& B, bool* 
       e.segs->count = segs;
       e.segs->has_pos64 = L.fp64;
       e.segs->max_abs = e.scene->light_max;
+      e.segs->nonneg = e.scene->light_nonneg;
       e.scene->seg_hint = std::max(e.scene->seg_hint, segs);
     }
   }

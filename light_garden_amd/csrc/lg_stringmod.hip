@@ -492,11 +492,16 @@ int lg_string_mod_draw(lg_ctx* ctx, uint32_t n, const lg_string_mod* mods, lg_se
   std::vector<SmDev> descs;
   std::vector<lg_mod_rem_color> mrc;
   uint64_t total = 0;
+  bool nonneg = true;  // a segment's colour is the StringMod's or a mean of its modulo colours
   for (uint32_t i = 0; i < n; i++) {
     const lg_string_mod& outer = mods[i];
     const lg_string_mod& m = outer.nested ? *outer.nested : outer;
     SmDev d = make_dev(m, (uint32_t)mrc.size());
     mrc.insert(mrc.end(), m.modulo_colors, m.modulo_colors + m.n_modulo_colors);
+    for (int c = 0; c < 4; c++) {
+      nonneg = nonneg && m.color[c] >= 0.0f;
+      for (uint32_t k = 0; k < m.n_modulo_colors; k++) nonneg = nonneg && m.modulo_colors[k].color[c] >= 0.0f;
+    }
     if (outer.nested) {
       if (S.crossings.size() <= n_nested) S.crossings.push_back(new lg::DevBuf());
       lg::DevBuf& buf = *S.crossings[n_nested++];
@@ -545,6 +550,7 @@ int lg_string_mod_draw(lg_ctx* ctx, uint32_t n, const lg_string_mod* mods, lg_se
   }
   out->count = total;
   out->max_abs = max_abs;
+  out->nonneg = nonneg;
   out->has_pos64 = false;
   lg::end_call(ctx);
   return LG_OK;

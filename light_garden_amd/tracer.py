@@ -237,6 +237,11 @@ class Context:
         """Replay frames from hipGraphs while the batch shape is unchanged (default on)."""
         abi.call("lg_ctx_set_graphs", self._h, 1 if on else 0)
 
+    def set_packed_accumulation(self, on: bool):
+        """Two colour channels per 64-bit LDS add for scenes whose deposits are all non-negative
+        (default on); off: one channel per add for every scene."""
+        abi.call("lg_ctx_set_packed_accumulation", self._h, 1 if on else 0)
+
     def tuning(self) -> dict:
         t = abi.LgTuning()
         abi.call("lg_ctx_get_tuning", self._h, C.byref(t))

@@ -208,3 +208,81 @@ def test_whole_frame_equals_separate_stages(ctx, pieces):
             x.close()
     finally:
         ctx.set_tuning(**before)
+
+
+def render_both(ctx, pos, c0, c1, width, height, bounds=(-1.0, 1.0, -1.0, 1.0)):
+    """The accumulator of the same segments with two channels per LDS add and with one."""
+    out = []
+    for packed in (True, False):
+        ctx.set_packed_accumulation(packed)
+        try:
+            img = gpu_accum(ctx, pos, c0, c1, width, height, bounds)
+            out.append((img.accum(), img.rgba16f()))
+        finally:
+            ctx.set_packed_accumulation(True)
+    return out
+
+
+@pytest.mark.parametrize("pieces", [None, (64, 64)], ids=["default_pieces", "split_tiles"])
+def test_packed_matches_one_channel_per_add(ctx, orc, pieces):
+    """Two channels per 64-bit LDS add (frames whose deposits are all non-negative) against one
+    channel per add on the same segments: a traced scene (uniform colours) and string-mod
+    segments (per-vertex colours), also with every tile split over many items. Both stay
+    within 1e-5 of each other and of the FP64 accumulation."""
+    before = ctx.tuning()
+    try:
+        if pieces:
+            ctx.set_tuning(min_piece=pieces[0], max_piece=pieces[1])
+        scene = configs.plain_small(num_rays=6000)
+        pos, col, _ = orc.trace_scene(scene)
+        (pa, _), (ua, _) = render_both(ctx, pos, col, None, 512, 512)
+        ref = orc.accumulate_f64(pos, col, None, abi.LgView(-1.0, 1.0, -1.0, 1.0, 512, 512))
+        assert orc.rel_l1(pa, ua) < 1e-5
+        assert orc.rel_l1(pa, ref) < 1e-5
+        spec = configs.StringModSpec(modulo=3000, mults=(2, 7), width=384, height=384)
+        pos, col = configs.string_mod_segments(spec)
+        (pa, _), (ua, _) = render_both(ctx, pos, col, col, 384, 384)
+        ref = orc.accumulate_f64(pos, col, col, abi.LgView(-1.0, 1.0, -1.0, 1.0, 384, 384))
+        assert orc.rel_l1(pa, ua) < 1e-5
+        assert orc.rel_l1(pa, ref) < 1e-5
+    finally:
+        ctx.set_tuning(**before)
+
+
+@pytest.mark.parametrize("pieces", [None, (32768, 32768)], ids=["split", "one_item"])
+def test_packed_fields_at_their_bound(ctx, pieces):
+    """20,000 unit-colour segments on one pixel row of one tile: every pair deposits the largest
+    value into the same pixels, which drives each 32-bit field to its bound (pairs of the item
+    times the largest deposit, below 2^32). The sum is exactly 20,000 per channel, in one
+    item and split over several."""
+    before = ctx.tuning()
+    try:
+        if pieces:
+            ctx.set_tuning(min_piece=pieces[0], max_piece=pieces[1])
+        n, side = 20000, 64
+        row = 17
+        y = 1.0 - (row + 0.5) * 2.0 / side   # a pixel-centre row: the whole weight on one pixel
+        pos = np.tile(np.array([[-1.0, y, 1.0, y]]), (n, 1))
+        col = np.ones((n, 4), np.float32)
+        (pa, _), (ua, _) = render_both(ctx, pos, col, None, side, side)
+        expect = np.zeros((side, side, 4), np.float32)
+        expect[row] = n
+        np.testing.assert_array_equal(pa, expect)
+        np.testing.assert_array_equal(ua, expect)
+    finally:
+        ctx.set_tuning(**before)
+
+
+def test_negative_deposits_take_one_channel_per_add(ctx, orc):
+    """A negative colour, and ReverseSubtract, make deposits negative: such frames accumulate one
+    channel per add and match the FP64 accumulation as before, bit for bit with packing off."""
+    rng = np.random.default_rng(12)
+    n = 3000
+    pos = rng.uniform(-1.1, 1.1, (n, 4)).astype(np.float32).astype(np.float64)
+    col = rng.uniform(0, 0.02, (n, 4)).astype(np.float32)
+    col[5, 1] = -0.01
+    (pa, pb), (ua, ub) = render_both(ctx, pos, col, None, 256, 256)
+    np.testing.assert_array_equal(pa, ua)
+    np.testing.assert_array_equal(pb, ub)
+    ref = orc.accumulate_f64(pos, col, None, abi.LgView(-1.0, 1.0, -1.0, 1.0, 256, 256))
+    assert orc.rel_l1(pa, ref) < 1e-5
