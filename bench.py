@@ -26,6 +26,10 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector
 # ds_add_u64 lane-operations per second over the whole chip, measured with a dedicated LDS
 # atomic loop on MI355X (no published figure exists); the LDS-atomic roof of tile_accum.
 LDS_U64_ATOMIC_PEAK = 6.0e12
+# every scene here deposits non-negative values only, so the tile walk keeps two colour
+# channels per 64-bit LDS word: 2 pixels x 2 words per anti-aliased sample
+LDS_ATOMICS_PER_SAMPLE = 4.0
+LDS_ATOMICS_NOTE = "4 LDS u64 atomics per anti-aliased sample (2 pixels x 2 words of two colour channels)"
 # Brute-force algorithmic FP32 flops per traced segment of the large scene: 16 circles x 18,
 # 16 rects x 28, 16 lenses x 56, 16 mirrors x 12, plus shading 40 and canvas exit 12.
 FLOPS_PER_SEGMENT_LARGE = 16 * (18 + 28 + 56 + 12) + 52
@@ -136,7 +140,7 @@ def tile_accum_roofline(stages, n_seg, px, seg_bytes=32.0, px_bytes=24.0):
     algo_bytes = seg_bytes * n_seg + px_bytes * px
     t_ta = ta["avg_launch_ms"] * 1e-3
     achieved = algo_bytes / t_ta / 1e9
-    lds_rate = 8.0 * ta["work_per_frame"] / t_ta
+    lds_rate = LDS_ATOMICS_PER_SAMPLE * ta["work_per_frame"] / t_ta
     return ({"bound": "hbm", "kernel": "tile_accum", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo_bytes,
              "avg_launch_ms": ta["avg_launch_ms"]},
@@ -477,13 +481,13 @@ def bench_scene(args, scene, desc, tuning_key, flops_per_segment, cpu_stride):
     roofline = {"bound": "hbm", "kernel": "tile_accum", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "algorithmic_bytes": algo_bytes, "avg_launch_ms": ta["avg_launch_ms"]}
-    lds_rate = 8.0 * samples / t_ta
+    lds_rate = LDS_ATOMICS_PER_SAMPLE * samples / t_ta
     trace_ms = sum(v["ms_per_frame"] for k, v in stages.items() if k.startswith("trace"))
     trace_tflops = flops_per_segment * n_seg / (trace_ms * 1e-3) / 1e12 if trace_ms else 0.0
     secondary = {
         "tile_accum_lds_atomics": {"achieved": lds_rate, "peak": LDS_U64_ATOMIC_PEAK, "unit": "ds_add_u64/s",
                                    "frac": lds_rate / LDS_U64_ATOMIC_PEAK, "samples_per_frame": samples,
-                                   "note": "8 LDS u64 atomics per anti-aliased sample (2 pixels x 4 channels)"},
+                                   "note": LDS_ATOMICS_NOTE},
         "trace_valu": {"achieved": trace_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": trace_tflops / FP32_PEAK_TFLOPS, "ms_per_frame": trace_ms,
                        "note": "brute-force algorithmic flops, %d per segment" % flops_per_segment},
