@@ -357,7 +357,9 @@ int lg_batch_render(lg_ctx* ctx, uint32_t n, lg_scene* const* scenes, const lg_p
 typedef struct lg_tuning {
   uint32_t rays_per_item;  /* rays per trace work item, [64, 2^20]; default 256 */
   uint32_t min_piece;      /* smallest tile work item in segment-tile pairs, >= 64; default 4096 */
-  uint32_t max_piece;      /* largest tile work item, >= min_piece; default 8192 */
+  uint32_t max_piece;      /* largest tile work item, >= min_piece; default 8192; work items
+                              hold at most 32768 pairs whatever the setting (the fixed point
+                              of two channels per LDS word is sized for that) */
   uint32_t trace_waves;    /* trace workgroups per compute unit, [1, 16]; default 4 */
   uint32_t bvh_min_objects; /* scenes with at least this many objects cull through their
                                bounding-volume hierarchy, others test every object;
