@@ -273,11 +273,21 @@ __global__ __launch_bounds__(256) void bin_kernel(const SceneDesc* __restrict__ 
     for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
       SegGeom g;
       if (!seg_geom(gen(D.seg_pos)[s], v, g)) continue;
+      // Consecutive segments mostly fall in the same tiles: in the count pass the lanes that share
+      // the tile of the first active lane add to its counter with one atomic, the others add their
+      // own (in the fill pass, where each lane needs its slot, that costs more than it saves).
       if (PASS == 0) {
-        for_each_tile(g, v, [&](int tile) { atomicAdd(&s_hist[tile], 1u); });
+        for_each_tile(g, v, [&](int tile) {
+          const int lt = __builtin_amdgcn_readfirstlane(tile);
+          const uint64_t same = __ballot(tile == lt);
+          if (tile != lt)
+            atomicAdd(&s_hist[tile], 1u);
+          else if (__lane_id() == (unsigned)__builtin_ctzll(same))
+            atomicAdd(&s_hist[lt], (uint32_t)__builtin_popcountll(same));
+        });
       } else {
         for_each_tile(g, v, [&](int tile) {
-          uint32_t slot = atomicAdd(&s_hist[tile], 1u);
+          const uint32_t slot = atomicAdd(&s_hist[tile], 1u);
           if (slot < pairs_cap) pairs[slot] = (uint32_t)s | ((uint32_t)g.xmajor << 31);
         });
       }
