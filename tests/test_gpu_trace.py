@@ -290,3 +290,39 @@ def test_python_trace_all_matches_c_entry(ctx, orc):
     rpos, rcol, _ = orc.trace_scene(scene)
     np.testing.assert_allclose(xy, rpos.reshape(-1, 2), atol=1e-9, rtol=0)
     np.testing.assert_array_equal(rgba, np.repeat(rcol, 2, axis=0))
+
+
+def test_trace_rejects_trees_beyond_four_primitives(ctx):
+    """Operand indices may share a node, which the compile unfolds: seven nodes whose unfolded
+    tree has six primitives are refused (LG_E_UNSUPPORTED); the same nodes without the sharing
+    (four primitives) trace."""
+    def tree(shared):
+        obj = abi.LgObject()
+        obj.has_material = 1
+        obj.refractive_index = 1.5
+        # 0: Or(1, 2); 1: And(3, 4); 2: And(3, 4) shared or And(5, 6); 3: Or(5, 6) or a circle
+        kids = {0: (1, 2), 1: (3, 4), 2: (3, 4) if shared else (5, 6)}
+        if shared:
+            kids[3] = (5, 6)
+        for i in range(7):
+            s = obj.shapes[i]
+            s.rotation[:] = (1.0, 0.0, 0.0, 1.0)
+            if i in kids:
+                s.kind, s.op = abi.SHAPE_LOGIC, abi.LOGIC_OR if i in (0, 3) else abi.LOGIC_AND
+                s.child_a, s.child_b = kids[i]
+            else:
+                s.kind = abi.SHAPE_CIRCLE
+                s.origin[0], s.origin[1] = 0.02 * i, 0.0
+                s.p[0] = 0.1
+        obj.n_shapes = 7
+        return obj
+
+    for shared, want in ((True, abi.LG_E_UNSUPPORTED), (False, abi.LG_OK)):
+        tr = lg.Tracer(ctx)
+        tr.push_light(lg.PointLight((0.5, 0.5), 100, [0.1, 0.1, 0.1, 0.1]))
+        tr.sync()
+        lib = abi.load()
+        assert lib.lg_scene_push_object(tr._h, C.byref(tree(shared))) == abi.LG_OK
+        segs = lg.Segments(ctx)
+        p = tr.params()
+        assert lib.lg_trace(tr._h, C.byref(p), segs.handle) == want
