@@ -791,10 +791,13 @@ __global__ __launch_bounds__(BLOCK) void tile_accum_kernel(const SceneDesc* __re
               q2 = pack_q(blend_value(bc2, __builtin_fmaf(sp, d2, a2), al, bk2), cs2);
               q3 = pack_q(blend_value(bc3, al, al, bk3), cs3);
             }
-            const uint32_t l0 = (uint32_t)__builtin_fmaf(wlf, (float)q0, 0.5f);
-            const uint32_t l1 = (uint32_t)__builtin_fmaf(wlf, (float)q1, 0.5f);
-            const uint32_t l2 = (uint32_t)__builtin_fmaf(wlf, (float)q2, 0.5f);
-            const uint32_t l3 = (uint32_t)__builtin_fmaf(wlf, (float)q3, 0.5f);
+            // round((1 - f) q) as the high word of q * w + 2^31, w = (1 - f) 2^32 (saturating at
+            // 2^32 - 1): one 64-bit multiply-add per channel, at most q
+            const uint64_t w32 = (uint32_t)(wlf * 4294967296.0f);
+            const uint32_t l0 = (uint32_t)(((uint64_t)q0 * w32 + 0x80000000ull) >> 32);
+            const uint32_t l1 = (uint32_t)(((uint64_t)q1 * w32 + 0x80000000ull) >> 32);
+            const uint32_t l2 = (uint32_t)(((uint64_t)q2 * w32 + 0x80000000ull) >> 32);
+            const uint32_t l3 = (uint32_t)(((uint64_t)q3 * w32 + 0x80000000ull) >> 32);
             if (lo_in) {
               atomicAdd(t, (unsigned long long)l0 | ((unsigned long long)l1 << 32));
               atomicAdd(t + kPlane, (unsigned long long)l2 | ((unsigned long long)l3 << 32));
